@@ -1,0 +1,377 @@
+"""DMRG / TDVP drivers over the device engine.
+
+The sweep structure restates the reference pure-Python backend so results
+match it bond for bond:
+
+* ``dmrg_sweep``    -- ``_dmrg_one_sweep`` (src/dmrgpy/pyitensor/dmrg.py:355-382):
+  right environments rebuilt from scratch, left-to-right then right-to-left,
+  energy = last local eigenvalue;
+* ``local_ground``  -- ``_local_ground_state`` (dmrg.py:302-333), Lanczos cap
+  ``max(niter, 200)``, tolerance 1e-12;
+* ``update``        -- ``_apply_local_update`` (dmrg.py:336-352);
+* ``tdvp_step``     -- ``tdvp_step`` / ``_half_sweep_lr`` / ``_half_sweep_rl``
+  (src/dmrgpy/pyitensor/tdvp.py:220-273, 350-368);
+* ``apply_mpo``     -- zip-up ``applyMPO`` (src/dmrgpy/pyitensor/mpsalgebra.py:226-302);
+* ``random_mps``    -- ``randomMPS`` (mpsalgebra.py:315-353), same draws from
+  the legacy global ``np.random``.
+
+Every contraction, Krylov step and SVD runs in libdmrg_hip.so; this module
+only sequences calls and keeps the device buffers.
+"""
+
+import math
+
+import numpy as np
+
+from .mps import DeviceMPS
+
+LANCZOS_TOL = 1e-12         # dmrg.py:95
+LANCZOS_CAP = 200           # dmrg.py:330
+KRYLOV_ERRGOAL = 1e-10      # tdvp.py:79
+KRYLOV_TOL = 1e-12          # tdvp.py:82
+
+
+class Stats:
+    """Counters of one run (matvecs per local solve, truncation)."""
+
+    def __init__(self):
+        self.lanczos_iters = []
+        self.krylov_iters = []
+        self.truncation = []      # (bond, keep, discarded weight)
+
+    def reset(self):
+        self.__init__()
+
+
+def ones_env(eng, field, prec):
+    t = eng.zeros((2 if field else 1, 1, 1, 1), dtype=eng.env_dtype(prec))
+    t[0] = 1.0
+    return t
+
+
+def theta2(eng, psi, i):
+    """A_i * A_{i+1} as a flat [np, Dl*d1*d2*Dr] FP64 vector (1-based bond i)."""
+    A, B = psi.A(i), psi.A(i + 1)
+    npl, Dl, d1, Dm = A.shape
+    _, _, d2, Dr = B.shape
+    th = eng.empty((npl, Dl * d1 * d2 * Dr))
+    eng.gemm(0, psi.field, A, B, th, Dl * d1, d2 * Dr, Dm, planeA=Dl * d1 * Dm, planeB=Dm * d2 * Dr,
+             planeC=Dl * d1 * d2 * Dr)
+    return th, (Dl, d1, d2, Dr)
+
+
+def env_left(eng, L, A, W, field, prec, bra=None):
+    bra = A if bra is None else bra
+    out = eng.empty((L.shape[0], W.shape[2], bra.shape[3], A.shape[3]), dtype=L.dtype)
+    eng.env_update("left", field, prec, A, bra, W, L, out)
+    return out
+
+
+def env_right(eng, R, A, W, field, prec, bra=None):
+    bra = A if bra is None else bra
+    out = eng.empty((R.shape[0], W.shape[1], A.shape[1], bra.shape[1]), dtype=R.dtype)
+    eng.env_update("right", field, prec, A, bra, W, R, out)
+    return out
+
+
+def right_envs(eng, psi, mpo, prec):
+    """{i: R_i}, R_i covering sites i..N (dmrg.py:204-211)."""
+    n = psi.length()
+    env = {n + 1: ones_env(eng, psi.field, prec)}
+    for i in range(n, 1, -1):
+        env[i] = env_right(eng, env[i + 1], psi.A(i), mpo.W[i - 1], psi.field, prec)
+    return env
+
+
+def left_envs(eng, psi, mpo, prec):
+    """{i: L_i}, L_i covering sites 1..i (dmrg.py:214-221)."""
+    n = psi.length()
+    env = {0: ones_env(eng, psi.field, prec)}
+    for i in range(1, n):
+        env[i] = env_left(eng, env[i - 1], psi.A(i), mpo.W[i - 1], psi.field, prec)
+    return env
+
+
+def heff2(eng, L, R, mpo, i, dims, field, prec):
+    Dl, d1, d2, Dr = dims
+    return eng.heff_desc(L, R, mpo.W[i - 1], mpo.W[i], Dl, Dr, d1, d2, field, prec)
+
+
+def heff1(eng, L, R, mpo, i, dims, field, prec):
+    Dl, d, Dr = dims
+    return eng.heff_desc(L, R, mpo.W[i - 1], None, Dl, Dr, d, 0, field, prec)
+
+
+def local_ground(eng, L, R, mpo, psi, i, niter, prec, stats=None, tol=LANCZOS_TOL, res_tol=0.0):
+    th, dims = theta2(eng, psi, i)
+    desc = heff2(eng, L, R, mpo, i, dims, psi.field, prec)
+    e, it = eng.lanczos(desc, th, max(niter, LANCZOS_CAP), tol, res_tol)
+    if stats is not None:
+        stats.lanczos_iters.append(it)
+    return e, th, dims
+
+
+def update(eng, psi, i, th, dims, cutoff, maxdim, direction, stats=None):
+    Dl, d1, d2, Dr = dims
+    npl = th.shape[0]
+    U, V, keep, disc, _ = eng.svd_split(psi.field, th, Dl * d1, d2 * Dr, cutoff, maxdim, 1,
+                                        0 if direction == "right" else 1)
+    psi.set_A(i, U.view(npl, Dl, d1, keep))
+    psi.set_A(i + 1, V.view(npl, keep, d2, Dr))
+    psi.center = i + 1 if direction == "right" else i
+    if stats is not None:
+        stats.truncation.append((i, keep, disc))
+    return keep, disc
+
+
+def dmrg_sweep(eng, psi, mpo, maxdim, cutoff, niter, prec=0, stats=None, tol=LANCZOS_TOL, res_tol=0.0):
+    n = psi.length()
+    energy = None
+    R = right_envs(eng, psi, mpo, prec)
+    L = {0: ones_env(eng, psi.field, prec)}
+    for i in range(1, n):
+        energy, th, dims = local_ground(eng, L[i - 1], R[i + 2], mpo, psi, i, niter, prec, stats, tol, res_tol)
+        update(eng, psi, i, th, dims, cutoff, maxdim, "right", stats)
+        L[i] = env_left(eng, L[i - 1], psi.A(i), mpo.W[i - 1], psi.field, prec)
+        R.pop(i + 2, None)
+    R = {n + 1: ones_env(eng, psi.field, prec)}
+    for i in range(n - 1, 0, -1):
+        energy, th, dims = local_ground(eng, L[i - 1], R[i + 2], mpo, psi, i, niter, prec, stats, tol, res_tol)
+        update(eng, psi, i, th, dims, cutoff, maxdim, "left", stats)
+        R[i + 1] = env_right(eng, R[i + 2], psi.A(i + 1), mpo.W[i], psi.field, prec)
+        L.pop(i, None)
+        R.pop(i + 2, None)
+    return energy
+
+
+def expectation(eng, bra, mpo, ket, prec=0):
+    """<bra|M|ket> by a full left-to-right environment sweep (mpsalgebra.py:99-114)."""
+    field = max(bra.field, ket.field, 1 if mpo.complex else 0)
+    bra = bra.as_complex() if field and not bra.field else bra
+    ket = ket.as_complex() if field and not ket.field else ket
+    L = ones_env(eng, field, prec)
+    for i in range(1, ket.length() + 1):
+        L = env_left(eng, L, ket.A(i), mpo.W[i - 1], field, prec, bra=bra.A(i))
+    v = L.double().cpu().numpy()
+    return complex(v[0, 0, 0, 0], v[1, 0, 0, 0] if field else 0.0)
+
+
+def identity_mpo(dims):
+    from .mpo import MPO
+    return MPO([np.eye(d)[None, None, None] for d in dims], codes=[None] * len(dims))
+
+
+def overlap(eng, bra, ket):
+    """<bra|ket> (mpsalgebra.py:89-98)."""
+    return expectation(eng, bra, identity_mpo(ket.phys_dims()), ket)
+
+
+def scale_site(psi, i, factor):
+    """Multiply the state by a scalar through site i (host-side complex
+    factor applied to the planes)."""
+    t = psi.A(i)
+    f = complex(factor)
+    if psi.field == 0:
+        if f.imag != 0:
+            raise ValueError("complex factor on a real MPS")
+        t.mul_(f.real)
+    else:
+        re, im = t[0].clone(), t[1].clone()
+        t[0] = f.real * re - f.imag * im
+        t[1] = f.real * im + f.imag * re
+
+
+def normalize(eng, psi):
+    nrm = math.sqrt(max(overlap(eng, psi, psi).real, 0.0))
+    scale_site(psi, psi.center or 1, 1.0 / nrm)
+    return psi
+
+
+def shift_right(eng, psi, i, cutoff=0.0, maxdim=None):
+    """mpscontainer.py:104-119 (1-based site i)."""
+    T, nxt = psi.A(i), psi.A(i + 1)
+    npl, Dl, d, Dr = T.shape
+    U, V, keep, _, _ = eng.svd_split(psi.field, T, Dl * d, Dr, cutoff, maxdim, 1, 0)
+    psi.set_A(i, U.view(npl, Dl, d, keep))
+    _, _, d2, Dr2 = nxt.shape
+    out = eng.empty((npl, keep, d2, Dr2))
+    eng.gemm(0, psi.field, V, nxt, out, keep, d2 * Dr2, Dr, planeA=keep * Dr, planeB=Dr * d2 * Dr2,
+             planeC=keep * d2 * Dr2)
+    psi.set_A(i + 1, out)
+
+
+def shift_left(eng, psi, i, cutoff=0.0, maxdim=None):
+    """mpscontainer.py:121-130 (1-based site i)."""
+    T, prv = psi.A(i), psi.A(i - 1)
+    npl, Dl, d, Dr = T.shape
+    U, V, keep, _, _ = eng.svd_split(psi.field, T, Dl, d * Dr, cutoff, maxdim, 1, 1)
+    psi.set_A(i, V.view(npl, keep, d, Dr))
+    _, Dl0, d0, _ = prv.shape
+    out = eng.empty((npl, Dl0, d0, keep))
+    eng.gemm(0, psi.field, prv, U, out, Dl0 * d0, keep, Dl, planeA=Dl0 * d0 * Dl, planeB=Dl * keep,
+             planeC=Dl0 * d0 * keep)
+    psi.set_A(i - 1, out)
+
+
+def position(eng, psi, b, cutoff=0.0, maxdim=None):
+    while psi.center < b:
+        shift_right(eng, psi, psi.center, cutoff, maxdim)
+        psi.center += 1
+    while psi.center > b:
+        shift_left(eng, psi, psi.center, cutoff, maxdim)
+        psi.center -= 1
+    return psi
+
+
+def random_bond_dims(dims, m):
+    n = len(dims)
+    bonds, cum = [], 1
+    for d in dims[:-1]:
+        cum = min(cum * d, m)
+        bonds.append(cum)
+    cum = 1
+    for k in range(n - 2, -1, -1):
+        cum = min(cum * dims[k + 1], m)
+        bonds[k] = min(bonds[k], cum)
+    return bonds
+
+
+def random_mps(eng, dims, m, field=1, rng=None):
+    """randomMPS (mpsalgebra.py:315-353): per site ``randn(*shape) +
+    1j*randn(*shape)`` from the legacy global numpy generator (or ``rng``),
+    reference boundary shapes; field 0 keeps the real part of the same
+    draws.  Canonicalised to centre N on the device and normalised."""
+    rs = np.random if rng is None else rng
+    n = len(dims)
+    bonds = random_bond_dims(dims, m)
+    arrays = []
+    for i in range(n):
+        shape = ([bonds[i - 1]] if i > 0 else []) + [dims[i]] + ([bonds[i]] if i < n - 1 else [])
+        a = rs.randn(*shape) + 1j * rs.randn(*shape)
+        Dl = bonds[i - 1] if i > 0 else 1
+        Dr = bonds[i] if i < n - 1 else 1
+        a = a.reshape(Dl, dims[i], Dr)
+        arrays.append(a if field else a.real)
+    psi = DeviceMPS.from_numpy(arrays, eng.device, field=field, center=1)
+    position(eng, psi, n)
+    normalize(eng, psi)
+    return psi
+
+
+# ---------------------------------------------------------------------------
+# MPO x MPS
+# ---------------------------------------------------------------------------
+def apply_mpo(eng, mpo, psi, cutoff=0.0, maxdim=None):
+    """Zip-up K|x> (mpsalgebra.py:226-302): per site contract the carried
+    remainder with the MPS and MPO tensors, split off a left-orthonormal U
+    with a truncated SVD, carry S*V on; centre N."""
+    field = max(psi.field, 1 if mpo.complex else 0)
+    x = psi.as_complex() if field and not psi.field else psi
+    npl = 2 if field else 1
+    n = x.length()
+    left = eng.zeros((npl, 1, 1, 1))          # [k, wl, Dl]
+    left[0] = 1.0
+    out = []
+    for i in range(1, n + 1):
+        X = x.A(i)
+        W = mpo.W[i - 1]                       # [np_w, wl, wr, s', s]
+        _, k, wl, Dl = left.shape
+        _, _, d, Dr = X.shape
+        wr = W.shape[2]
+        # T1[(k wl)][(s Dr)] = left[(k wl)][Dl] X[Dl][(s Dr)]
+        T1 = eng.empty((npl, k * wl * d * Dr))
+        eng.gemm(0, field, left, X, T1, k * wl, d * Dr, Dl, planeA=k * wl * Dl, planeB=Dl * d * Dr,
+                 planeC=k * wl * d * Dr)
+        # T2[k][(s' wr)][Dr] = Wm[(s' wr)][(wl s)] T1[k][(wl s)][Dr]
+        Wc = W[0] + (1j * W[1] if W.shape[0] == 2 else 0)
+        Wm = Wc.transpose(2, 1, 0, 3).reshape(d * wr, wl * d)
+        Wd = eng.torch.from_numpy(np.ascontiguousarray(
+            np.stack([Wm.real] + ([Wm.imag] if field else [])), dtype=np.float64)).to(eng.device)
+        T2 = eng.empty((npl, k * d * wr * Dr))
+        eng.gemm(0, field, Wd, T1, T2, d * wr, Dr, wl * d, strideA=0, strideB=wl * d * Dr,
+                 strideC=d * wr * Dr, planeA=d * wr * wl * d, planeB=k * wl * d * Dr,
+                 planeC=k * d * wr * Dr, batch=k)
+        if i == n:
+            out.append(T2.view(npl, k, d, wr * Dr))
+            break
+        U, V, keep, _, _ = eng.svd_split(field, T2, k * d, wr * Dr, cutoff, maxdim, 1, 0)
+        out.append(U.view(npl, k, d, keep))
+        left = V.view(npl, keep, wr, Dr)
+    return DeviceMPS(out, center=n, field=field)
+
+
+def apply_local(eng, psi, site, M):
+    """Exact single-site operator application (standard-convention matrix
+    M[out, in]) without changing the gauge."""
+    field = max(psi.field, 1 if np.iscomplexobj(M) and np.abs(np.imag(M)).max() > 0 else 0)
+    p = psi.as_complex() if field and not psi.field else psi.copy()
+    A = p.A(site)
+    npl, Dl, d, Dr = A.shape
+    Mc = np.asarray(M, dtype=complex)
+    Md = eng.torch.from_numpy(np.ascontiguousarray(
+        np.stack([Mc.real] + ([Mc.imag] if field else [])), dtype=np.float64)).to(eng.device)
+    out = eng.empty(A.shape)
+    eng.gemm(0, field, Md, A, out, d, Dr, d, strideA=0, strideB=d * Dr, strideC=d * Dr,
+             planeA=d * d, planeB=Dl * d * Dr, planeC=Dl * d * Dr, batch=Dl)
+    p.set_A(site, out)
+    return p
+
+
+# ---------------------------------------------------------------------------
+# TDVP
+# ---------------------------------------------------------------------------
+def _evolve2(eng, L, R, mpo, psi, i, coeff, niter, prec, stats):
+    th, dims = theta2(eng, psi, i)
+    desc = heff2(eng, L, R, mpo, i, dims, psi.field, prec)
+    it = eng.expm(desc, th, coeff, niter, KRYLOV_TOL, KRYLOV_ERRGOAL)
+    if stats is not None:
+        stats.krylov_iters.append(it)
+    return th, dims
+
+
+def _evolve1(eng, L, R, mpo, C, i, coeff, niter, prec, stats):
+    npl, Dl, d, Dr = C.shape
+    x = C.reshape(npl, Dl * d * Dr).clone()
+    desc = heff1(eng, L, R, mpo, i, (Dl, d, Dr), 1, prec)
+    it = eng.expm(desc, x, coeff, niter, KRYLOV_TOL, KRYLOV_ERRGOAL)
+    if stats is not None:
+        stats.krylov_iters.append(it)
+    return x.view(npl, Dl, d, Dr)
+
+
+def half_sweep_lr(eng, psi, mpo, tau, cutoff, maxdim, niter, prec=0, stats=None):
+    n = psi.length()
+    R = right_envs(eng, psi, mpo, prec)
+    L = {0: ones_env(eng, 1, prec)}
+    for i in range(1, n):
+        th, dims = _evolve2(eng, L[i - 1], R[i + 2], mpo, psi, i, -1j * tau, niter, prec, stats)
+        update(eng, psi, i, th, dims, cutoff, maxdim, "right", stats)
+        L[i] = env_left(eng, L[i - 1], psi.A(i), mpo.W[i - 1], 1, prec)
+        if i < n - 1:
+            psi.set_A(i + 1, _evolve1(eng, L[i], R[i + 2], mpo, psi.A(i + 1), i + 1, 1j * tau, niter, prec,
+                                      stats))
+    psi.center = n
+
+
+def half_sweep_rl(eng, psi, mpo, tau, cutoff, maxdim, niter, prec=0, stats=None):
+    n = psi.length()
+    L = left_envs(eng, psi, mpo, prec)
+    R = {n + 1: ones_env(eng, 1, prec)}
+    for i in range(n - 1, 0, -1):
+        th, dims = _evolve2(eng, L[i - 1], R[i + 2], mpo, psi, i, -1j * tau, niter, prec, stats)
+        update(eng, psi, i, th, dims, cutoff, maxdim, "left", stats)
+        R[i + 1] = env_right(eng, R[i + 2], psi.A(i + 1), mpo.W[i], 1, prec)
+        if i > 1:
+            psi.set_A(i, _evolve1(eng, L[i - 1], R[i + 1], mpo, psi.A(i), i, 1j * tau, niter, prec, stats))
+    psi.center = 1
+
+
+def tdvp_step(eng, psi, mpo, dt, cutoff, maxdim, niter=50, prec=0, stats=None):
+    """One step exp(-i dt H) (tdvp.py:350-368, two-site); psi is made complex."""
+    if not psi.field:
+        c = psi.as_complex()
+        psi.tensors, psi.field = c.tensors, 1
+    tau = dt / 2.0
+    half_sweep_lr(eng, psi, mpo, tau, cutoff, maxdim, niter, prec, stats)
+    half_sweep_rl(eng, psi, mpo, tau, cutoff, maxdim, niter, prec, stats)
+    return psi

@@ -1,0 +1,249 @@
+// extern "C" entry points of libdmrg_hip.so (declared in include/dmrg_hip.h).
+// Every entry point catches library errors and returns a status code; the
+// message is kept per thread for dmrg_last_error().
+#include <complex>
+#include <cstring>
+
+#include "dmrg_internal.h"
+
+struct dmrg_ctx {
+  dmrg::Ctx c;
+};
+
+namespace dmrg {
+
+void lanczos_ground(Ctx* c, const HeffOp& h, double* x, int maxiter, double tol, double res_tol, double* energy,
+                    int* iters);
+void krylov_expm(Ctx* c, const HeffOp& h, double* x, std::complex<double> coeff, int maxiter, double tol,
+                 double errgoal, int* iters);
+double rayleigh(Ctx* c, const HeffOp& h, const double* x);
+int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
+                    int absorb, double* U, double* V, int kcap, int* keep, double* discarded, double* svals_host);
+
+void* ws_get(Ctx* c, Slot s, size_t bytes) { return c->ws[s].get(bytes, c->stream); }
+
+static hipEvent_t pool_event(Ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  DMRG_CHECK_HIP(hipEventCreate(&e));
+  return e;
+}
+
+// Folds finished event pairs into the per-kind totals once the list grows.
+static void prof_drain(Ctx* c, bool all) {
+  if (c->prof.empty()) return;
+  if (all) DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  std::vector<ProfEntry> keep;
+  for (auto& p : c->prof) {
+    if (!all && hipEventQuery(p.stop) != hipSuccess) {
+      keep.push_back(p);
+      continue;
+    }
+    float ms = 0;
+    DMRG_CHECK_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
+    c->prof_time_ms[p.kind] += ms;
+    c->prof_flops[p.kind] += p.flops;
+    c->prof_calls[p.kind] += 1;
+    c->event_pool.push_back(p.start);
+    c->event_pool.push_back(p.stop);
+  }
+  c->prof.swap(keep);
+}
+
+ProfScope::ProfScope(Ctx* c_, int kind_, double flops_) : c(c_), kind(kind_), flops(flops_) {
+  if (!c->prof_on) return;
+  a = pool_event(c);
+  b = pool_event(c);
+  DMRG_CHECK_HIP(hipEventRecord(a, c->stream));
+}
+ProfScope::~ProfScope() {
+  if (!a) return;
+  if (hipEventRecord(b, c->stream) != hipSuccess) return;
+  c->prof.push_back(ProfEntry{a, b, flops, kind});
+  if (c->prof.size() > 4096) {
+    try {
+      prof_drain(c, false);
+    } catch (...) {
+    }
+  }
+}
+
+}  // namespace dmrg
+
+using dmrg::Error;
+
+static thread_local std::string g_err;
+
+#define API_BEGIN try {
+#define API_END                     \
+  return 0;                         \
+  }                                 \
+  catch (const Error& e) {          \
+    g_err = e.msg;                  \
+    return -1;                      \
+  }                                 \
+  catch (const std::exception& e) { \
+    g_err = e.what();               \
+    return -2;                      \
+  }                                 \
+  catch (...) {                     \
+    g_err = "unknown error";        \
+    return -3;                      \
+  }
+
+extern "C" {
+
+int dmrg_version(void) { return 100; }
+
+const char* dmrg_last_error(void) { return g_err.c_str(); }
+
+int dmrg_ctx_create(int device, void* stream, dmrg_ctx** out) {
+  API_BEGIN
+  DMRG_REQUIRE(out != nullptr, "ctx_create: null out");
+  int ndev = 0;
+  DMRG_CHECK_HIP(hipGetDeviceCount(&ndev));
+  DMRG_REQUIRE(device >= 0 && device < ndev, "ctx_create: no such device");
+  DMRG_CHECK_HIP(hipSetDevice(device));
+  dmrg_ctx* h = new dmrg_ctx();
+  h->c.device = device;
+  h->c.stream = (hipStream_t)stream;
+  if (rocblas_create_handle(&h->c.blas) != rocblas_status_success) {
+    delete h;
+    throw Error("rocblas_create_handle failed");
+  }
+  rocblas_set_stream(h->c.blas, h->c.stream);
+  DMRG_CHECK_HIP(hipHostMalloc((void**)&h->c.host_scalars, 64 * sizeof(double), hipHostMallocDefault));
+  std::memset(h->c.host_scalars, 0, 64 * sizeof(double));
+  *out = h;
+  API_END
+}
+
+int dmrg_ctx_destroy(dmrg_ctx* ctx) {
+  API_BEGIN
+  if (!ctx) return 0;
+  dmrg::Ctx& c = ctx->c;
+  (void)hipStreamSynchronize(c.stream);
+  for (auto& p : c.prof) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  for (auto e : c.event_pool) (void)hipEventDestroy(e);
+  for (int i = 0; i < dmrg::WS_NSLOTS; ++i) c.ws[i].release();
+  if (c.host_scalars) (void)hipHostFree(c.host_scalars);
+  if (c.blas) rocblas_destroy_handle(c.blas);
+  delete ctx;
+  API_END
+}
+
+int dmrg_sync(dmrg_ctx* ctx) {
+  API_BEGIN
+  DMRG_CHECK_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END
+}
+
+int dmrg_heff_apply(dmrg_ctx* ctx, const dmrg_heff_desc* op, const double* x, double* y) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && op && x && y, "heff_apply: null argument");
+  dmrg::HeffOp h = dmrg::heff_prepare(&ctx->c, *op);
+  dmrg::heff_apply(&ctx->c, h, x, y);
+  API_END
+}
+
+int dmrg_rayleigh(dmrg_ctx* ctx, const dmrg_heff_desc* op, const double* x, double* value) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && op && x && value, "rayleigh: null argument");
+  dmrg::HeffOp h = dmrg::heff_prepare(&ctx->c, *op);
+  *value = dmrg::rayleigh(&ctx->c, h, x);
+  API_END
+}
+
+int dmrg_env_update(dmrg_ctx* ctx, const dmrg_env_desc* env) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && env && env->ket && env->bra && env->W && env->env_in && env->env_out,
+               "env_update: null argument");
+  dmrg::env_update(&ctx->c, *env);
+  API_END
+}
+
+int dmrg_lanczos_ground(dmrg_ctx* ctx, const dmrg_heff_desc* op, double* x, int maxiter, double tol,
+                        double res_tol, double* energy, int* iters) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && op && x && energy, "lanczos: null argument");
+  dmrg::HeffOp h = dmrg::heff_prepare(&ctx->c, *op);
+  dmrg::lanczos_ground(&ctx->c, h, x, maxiter, tol, res_tol, energy, iters);
+  API_END
+}
+
+int dmrg_krylov_expm(dmrg_ctx* ctx, const dmrg_heff_desc* op, double* x, double cre, double cim, int maxiter,
+                     double tol, double errgoal, int* iters) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && op && x, "expm: null argument");
+  dmrg::HeffOp h = dmrg::heff_prepare(&ctx->c, *op);
+  dmrg::krylov_expm(&ctx->c, h, x, std::complex<double>(cre, cim), maxiter, tol, errgoal, iters);
+  API_END
+}
+
+int dmrg_svd_split(dmrg_ctx* ctx, int field, const double* M, int m, int n, double cutoff, int maxdim,
+                   int mindim, int absorb, double* U, double* V, int kcap, int* keep, double* discarded,
+                   double* svals_host) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && M && U && V && keep, "svd_split: null argument");
+  DMRG_REQUIRE(field == 0 || field == 1, "svd_split: bad field");
+  dmrg::svd_split_entry(&ctx->c, field, M, m, n, cutoff, maxdim, mindim, absorb, U, V, kcap, keep, discarded,
+                        svals_host);
+  API_END
+}
+
+int dmrg_gemm(dmrg_ctx* ctx, int prec, int field, int transA, int transB, int conjA, int conjB, int M, int N,
+              int K, double alpha, const void* A, int lda, long long strideA, long long planeA, const void* B,
+              int ldb, long long strideB, long long planeB, double beta, void* C, int ldc, long long strideC,
+              long long planeC, int batch) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && A && B && C, "gemm: null argument");
+  DMRG_REQUIRE(prec >= 0 && prec <= 2 && (field == 0 || field == 1), "gemm: bad prec/field");
+  dmrg::GemmDesc g = dmrg::gdesc(M, N, K, lda, ldb, ldc, field, 0);
+  g.transA = transA != 0;
+  g.transB = transB != 0;
+  g.conjA = conjA != 0;
+  g.conjB = conjB != 0;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.sA = strideA; g.sB = strideB; g.sC = strideC;
+  g.pA = planeA; g.pB = planeB; g.pC = planeC;
+  g.batch = batch;
+  dmrg::gemm_prec(&ctx->c, prec, g, A, B, C);
+  API_END
+}
+
+int dmrg_prof_enable(dmrg_ctx* ctx, int on) {
+  API_BEGIN
+  dmrg::Ctx& c = ctx->c;
+  dmrg::prof_drain(&c, true);
+  for (int i = 0; i < 4; ++i) {
+    c.prof_time_ms[i] = 0;
+    c.prof_flops[i] = 0;
+    c.prof_calls[i] = 0;
+  }
+  c.prof_on = on != 0;
+  API_END
+}
+
+int dmrg_prof_read(dmrg_ctx* ctx, double* time_ms, double* flops, long long* calls) {
+  API_BEGIN
+  dmrg::Ctx& c = ctx->c;
+  dmrg::prof_drain(&c, true);
+  for (int i = 0; i < 4; ++i) {
+    if (time_ms) time_ms[i] = c.prof_time_ms[i];
+    if (flops) flops[i] = c.prof_flops[i];
+    if (calls) calls[i] = c.prof_calls[i];
+  }
+  API_END
+}
+
+long long dmrg_matvec_count(dmrg_ctx* ctx) { return ctx ? ctx->c.matvecs : -1; }
+
+}  // extern "C"

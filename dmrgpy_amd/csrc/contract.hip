@@ -1,0 +1,421 @@
+// Effective-Hamiltonian apply and environment updates, built as ordered
+// chains of MFMA GEMMs plus one HBM-bound "small operator" pass that applies
+// the MPO tensors (K = w*d <= ~100) between them.
+//
+// Layouts (row-major; complex = two planes):
+//   MPS site      A[Dl][d][Dr]                       (FP64)
+//   MPO site      W[wl][wr][s_out][s_in]             (FP64, host)
+//   left env      L[w][D_bra][D_ket]                 (contraction precision)
+//   right env     R[w][D_ket][D_bra]                 (contraction precision)
+//   two-site vec  theta[Dl][d1][d2][Dr]              (FP64)
+//
+// H_eff two-site (reference operand order L, W_i, W_{i+1}, R):
+//   T1[(wl a')][(p b)]  = L[(wl a')][a] * theta[a][(p b)]          GEMM
+//   T2[a'][(p' wr)][b]  = sum_{wl,p} W12[(p' wr)][(wl p)] T1[...]  small-op
+//   y[(a' p')][b']      = T2[(a' p')][(wr b)] * R[(wr b)][b']      GEMM
+// with W12 = sum_wm W_i W_{i+1} pre-contracted on the host once per bond.
+#include "dmrg_common.h"
+#include "dmrg_internal.h"
+
+#include <cstring>
+
+namespace dmrg {
+
+// ---------------------------------------------------------------------------
+// small operator: out[x][o][inner] = sum_i Wsm[o][i] * in[x][i][inner], with
+// the (o, i) -> memory offset maps given by ooff / ioff.  One thread per
+// (x, inner) element and a chunk of OCH outputs; W is uniform across the
+// wave and is read through the scalar cache.
+// ---------------------------------------------------------------------------
+constexpr int OCH = 16;
+
+template <typename T, bool CXV, bool CXW>
+__global__ __launch_bounds__(256) void smallop_k(const T* __restrict__ in, T* __restrict__ out,
+                                                 const T* __restrict__ Wsm, const int* __restrict__ ioff,
+                                                 const int* __restrict__ ooff, int X, int inner,
+                                                 long long sx_in, long long sx_out, int Kin, int Kout,
+                                                 int KoutPad, long long pin, long long pout) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = blockIdx.y;
+  const int oc = blockIdx.z * OCH;
+  if (b >= inner) return;
+  const T* ip = in + (long long)x * sx_in + b;
+  T* op = out + (long long)x * sx_out + b;
+  const long long wplane = (long long)KoutPad * Kin;
+  T ar[OCH], ai[OCH];
+#pragma unroll
+  for (int o = 0; o < OCH; ++o) { ar[o] = T(0); ai[o] = T(0); }
+  for (int i = 0; i < Kin; ++i) {
+    const long long off = ioff[i];
+    const T vr = ip[off];
+    const T vi = CXV ? ip[off + pin] : T(0);
+    const T* wrow = Wsm + (long long)oc * Kin + i;
+#pragma unroll
+    for (int o = 0; o < OCH; ++o) {
+      const T w = wrow[(long long)o * Kin];
+      ar[o] += w * vr;
+      if (CXV) ai[o] += w * vi;
+      if (CXW) {
+        const T wi = wrow[(long long)o * Kin + wplane];
+        ar[o] -= wi * vi;
+        ai[o] += wi * vr;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < OCH; ++o) {
+    if (oc + o < Kout) {
+      const long long off = ooff[oc + o];
+      op[off] = ar[o];
+      if (CXV) op[off + pout] = ai[o];
+    }
+  }
+}
+
+template <typename T>
+__global__ void convert_k(const double* __restrict__ src, T* __restrict__ dst, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = (T)src[i];
+}
+template <typename T>
+__global__ void convert_back_k(const T* __restrict__ src, double* __restrict__ dst, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = (double)src[i];
+}
+
+static unsigned grid1d(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+void to_prec(Ctx* c, int prec, const double* src, void* dst, long long n) {
+  if (n <= 0) return;
+  if (prec == P64) {
+    if (src != dst)
+      DMRG_CHECK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    convert_k<float><<<grid1d(n), 256, 0, c->stream>>>(src, (float*)dst, n);
+    DMRG_CHECK_HIP(hipGetLastError());
+  }
+}
+void from_prec(Ctx* c, int prec, const void* src, double* dst, long long n) {
+  if (n <= 0) return;
+  if (prec == P64) {
+    if (src != dst)
+      DMRG_CHECK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    convert_back_k<float><<<grid1d(n), 256, 0, c->stream>>>((const float*)src, dst, n);
+    DMRG_CHECK_HIP(hipGetLastError());
+  }
+}
+
+size_t prec_size(int prec) { return prec == P64 ? sizeof(double) : sizeof(float); }
+
+// Uploads a small operator (W rows padded to a multiple of OCH) and its
+// offset tables into a workspace slot; returns device pointers.
+SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& wre,
+                       const std::vector<double>& wim, bool wcplx, int Kin, int Kout,
+                       const std::vector<int>& ioff, const std::vector<int>& ooff) {
+  SmallOp s;
+  s.Kin = Kin;
+  s.Kout = Kout;
+  s.KoutPad = ((Kout + OCH - 1) / OCH) * OCH;
+  s.wcplx = wcplx;
+  const size_t es = prec_size(prec);
+  const size_t nw = (size_t)s.KoutPad * Kin * (wcplx ? 2 : 1);
+  const size_t bytes_w = ((nw * es + 255) / 256) * 256;
+  const size_t bytes = bytes_w + sizeof(int) * (Kin + Kout) + 256;
+  std::vector<unsigned char> host(bytes, 0);
+  for (int pl = 0; pl < (wcplx ? 2 : 1); ++pl) {
+    const std::vector<double>& src = pl ? wim : wre;
+    for (int o = 0; o < Kout; ++o)
+      for (int i = 0; i < Kin; ++i) {
+        size_t idx = (size_t)pl * s.KoutPad * Kin + (size_t)o * Kin + i;
+        double v = src[(size_t)o * Kin + i];
+        if (prec == P64) reinterpret_cast<double*>(host.data())[idx] = v;
+        else reinterpret_cast<float*>(host.data())[idx] = (float)v;
+      }
+  }
+  std::memcpy(host.data() + bytes_w, ioff.data(), sizeof(int) * Kin);
+  std::memcpy(host.data() + bytes_w + sizeof(int) * Kin, ooff.data(), sizeof(int) * Kout);
+  unsigned char* dev = (unsigned char*)ws_get(c, slot, bytes);
+  DMRG_CHECK_HIP(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  // the host vector dies at return: make the copy complete first
+  DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  s.W = dev;
+  s.ioff = reinterpret_cast<const int*>(dev + bytes_w);
+  s.ooff = reinterpret_cast<const int*>(dev + bytes_w + sizeof(int) * Kin);
+  return s;
+}
+
+void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, void* out, int X,
+                 int inner, long long sx_in, long long sx_out, long long pin, long long pout) {
+  if (X <= 0 || inner <= 0 || s.Kout <= 0) return;
+  DMRG_REQUIRE(X <= 65535, "smallop: outer extent too large");
+  dim3 grid((inner + 255) / 256, X, s.KoutPad / OCH);
+  const bool cv = field == CPLX;
+#define LAUNCH(T, CV, CW)                                                                         \
+  smallop_k<T, CV, CW><<<grid, 256, 0, c->stream>>>((const T*)in, (T*)out, (const T*)s.W, s.ioff, \
+                                                     s.ooff, X, inner, sx_in, sx_out, s.Kin,     \
+                                                     s.Kout, s.KoutPad, pin, pout)
+  if (prec == P64) {
+    if (!cv) { DMRG_REQUIRE(!s.wcplx, "smallop: complex MPO needs complex vectors"); LAUNCH(double, false, false); }
+    else if (s.wcplx) LAUNCH(double, true, true);
+    else LAUNCH(double, true, false);
+  } else {
+    if (!cv) { DMRG_REQUIRE(!s.wcplx, "smallop: complex MPO needs complex vectors"); LAUNCH(float, false, false); }
+    else if (s.wcplx) LAUNCH(float, true, true);
+    else LAUNCH(float, true, false);
+  }
+#undef LAUNCH
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
+void gemm_prec(Ctx* c, int prec, const GemmDesc& g, const void* A, const void* B, void* C) {
+  if (prec == P64) gemm_f64(c, g, (const double*)A, (const double*)B, (double*)C);
+  else if (prec == P32) gemm_f32(c, g, (const float*)A, (const float*)B, (float*)C);
+  else gemm_bf16x3(c, g, (const float*)A, (const float*)B, (float*)C);
+}
+
+GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag) {
+  GemmDesc g;
+  g.M = M; g.N = N; g.K = K;
+  g.lda = lda; g.ldb = ldb; g.ldc = ldc;
+  g.sA = g.sB = g.sC = 0;
+  g.pA = g.pB = g.pC = 0;
+  g.batch = 1;
+  g.transA = g.transB = false;
+  g.conjA = g.conjB = false;
+  g.alpha = 1.0; g.beta = 0.0;
+  g.field = field;
+  g.tag = tag;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// H_eff
+// ---------------------------------------------------------------------------
+static double wget(const double* W, int i0, int i1, int i2, int i3, int n1, int n2, int n3) {
+  return W[(((size_t)i0 * n1 + i1) * n2 + i2) * n3 + i3];
+}
+
+HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
+  HeffOp h;
+  h.d = d;
+  DMRG_REQUIRE(d.nsite == 1 || d.nsite == 2, "heff: nsite must be 1 or 2");
+  DMRG_REQUIRE(d.Dl > 0 && d.Dr > 0 && d.d1 > 0 && d.wl > 0 && d.wr > 0, "heff: bad dims");
+  DMRG_REQUIRE(d.field == REAL || d.field == CPLX, "heff: bad field");
+  DMRG_REQUIRE(d.prec == P64 || d.prec == P32 || d.prec == PB3, "heff: bad precision");
+  DMRG_REQUIRE(!(d.w_complex && d.field == REAL), "heff: complex MPO requires complex field");
+  const int Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr;
+  const bool wc = d.w_complex != 0;
+  if (d.nsite == 2) {
+    DMRG_REQUIRE(d.d2 > 0 && d.wm > 0, "heff: bad two-site dims");
+    const int d1 = d.d1, d2 = d.d2, wm = d.wm, P = d1 * d2;
+    const int Kin = wl * P, Kout = P * wr;
+    std::vector<double> wre((size_t)Kout * Kin, 0.0), wim(wc ? (size_t)Kout * Kin : 0, 0.0);
+    const size_t n1 = (size_t)wl * wm * d1 * d1, n2 = (size_t)wm * wr * d2 * d2;
+    for (int a = 0; a < wl; ++a)
+      for (int b = 0; b < wr; ++b)
+        for (int s1o = 0; s1o < d1; ++s1o)
+          for (int s2o = 0; s2o < d2; ++s2o)
+            for (int s1 = 0; s1 < d1; ++s1)
+              for (int s2 = 0; s2 < d2; ++s2) {
+                double re = 0, im = 0;
+                for (int m = 0; m < wm; ++m) {
+                  double ar = wget(d.W1, a, m, s1o, s1, wm, d1, d1);
+                  double br = wget(d.W2, m, b, s2o, s2, wr, d2, d2);
+                  double ai = wc ? wget(d.W1 + n1, a, m, s1o, s1, wm, d1, d1) : 0;
+                  double bi = wc ? wget(d.W2 + n2, m, b, s2o, s2, wr, d2, d2) : 0;
+                  re += ar * br - ai * bi;
+                  im += ar * bi + ai * br;
+                }
+                int o = (s1o * d2 + s2o) * wr + b, i = a * P + s1 * d2 + s2;
+                wre[(size_t)o * Kin + i] = re;
+                if (wc) wim[(size_t)o * Kin + i] = im;
+              }
+    std::vector<int> ioff(Kin), ooff(Kout);
+    for (int a = 0; a < wl; ++a)
+      for (int p = 0; p < P; ++p) ioff[a * P + p] = a * Dl * P * Dr + p * Dr;
+    for (int p = 0; p < P; ++p)
+      for (int b = 0; b < wr; ++b) ooff[p * wr + b] = (p * wr + b) * Dr;
+    h.sop = upload_smallop(c, WS_WSMALL, d.prec, wre, wim, wc, Kin, Kout, ioff, ooff);
+    h.P = P;
+  } else {
+    const int dd = d.d1, P = dd;
+    const int Kin = wl * P, Kout = P * wr;
+    std::vector<double> wre((size_t)Kout * Kin, 0.0), wim(wc ? (size_t)Kout * Kin : 0, 0.0);
+    const size_t n1 = (size_t)wl * wr * dd * dd;
+    for (int a = 0; a < wl; ++a)
+      for (int b = 0; b < wr; ++b)
+        for (int so = 0; so < dd; ++so)
+          for (int s = 0; s < dd; ++s) {
+            int o = so * wr + b, i = a * P + s;
+            wre[(size_t)o * Kin + i] = wget(d.W1, a, b, so, s, wr, dd, dd);
+            if (wc) wim[(size_t)o * Kin + i] = wget(d.W1 + n1, a, b, so, s, wr, dd, dd);
+          }
+    std::vector<int> ioff(Kin), ooff(Kout);
+    for (int a = 0; a < wl; ++a)
+      for (int p = 0; p < P; ++p) ioff[a * P + p] = a * Dl * P * Dr + p * Dr;
+    for (int p = 0; p < P; ++p)
+      for (int b = 0; b < wr; ++b) ooff[p * wr + b] = (p * wr + b) * Dr;
+    h.sop = upload_smallop(c, WS_WSMALL, d.prec, wre, wim, wc, Kin, Kout, ioff, ooff);
+    h.P = P;
+  }
+  h.n = (long long)Dl * h.P * Dr;
+  // offsets in the small-op tables are 32-bit
+  DMRG_REQUIRE((long long)wl * Dl * h.P * Dr < (1LL << 31), "heff: tensor too large for 32-bit offsets");
+  h.flops_per_apply = heff_flops(d);
+  return h;
+}
+
+double heff_flops(const dmrg_heff_desc& d) {
+  const double Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr;
+  double mac;
+  if (d.nsite == 2) {
+    const double d1 = d.d1, d2 = d.d2, wm = d.wm;
+    mac = Dl * Dl * wl * d1 * d2 * Dr + Dl * Dr * d2 * wl * wm * d1 * d1 + Dl * Dr * d1 * wm * wr * d2 * d2 +
+          Dl * d1 * d2 * wr * Dr * Dr;
+  } else {
+    const double dd = d.d1;
+    mac = Dl * Dl * wl * dd * Dr + Dl * Dr * wl * wr * dd * dd + Dl * dd * wr * Dr * Dr;
+  }
+  return (d.field == CPLX ? 8.0 : 2.0) * mac;
+}
+
+// y = H_eff x, x and y FP64 (planar if complex), length h.n (per plane).
+void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
+  const dmrg_heff_desc& d = h.d;
+  const int Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr, P = h.P;
+  const int np = d.field == CPLX ? 2 : 1;
+  const size_t es = prec_size(d.prec);
+  const long long n = h.n;
+  const long long nT = (long long)wl * Dl * P * Dr;   // T1 per plane
+  const long long nT2 = (long long)Dl * P * wr * Dr;  // T2 per plane
+  ProfScope ps(c, 0, h.flops_per_apply);
+  const void* xin = x;
+  void* yout = y;
+  if (d.prec != P64) {
+    void* xb = ws_get(c, WS_XIN, n * np * es);
+    to_prec(c, d.prec, x, xb, n * np);
+    xin = xb;
+    yout = ws_get(c, WS_YOUT, n * np * es);
+  }
+  void* T1 = ws_get(c, WS_T1, nT * np * es);
+  void* T2 = ws_get(c, WS_T2, nT2 * np * es);
+  {  // T1[(wl a')][(p b)] = L[(wl a')][a] * x[a][(p b)]
+    GemmDesc g = gdesc(wl * Dl, P * Dr, Dl, Dl, P * Dr, P * Dr, d.field, 1);
+    g.pA = (long long)wl * Dl * Dl; g.pB = n; g.pC = nT;
+    gemm_prec(c, d.prec, g, d.L, xin, T1);
+  }
+  run_smallop(c, d.prec, d.field, h.sop, T1, T2, Dl, Dr, (long long)P * Dr, (long long)P * wr * Dr, nT, nT2);
+  {  // y[(a' p')][b'] = T2[(a' p')][(wr b)] * R[(wr b)][b']
+    GemmDesc g = gdesc(Dl * P, Dr, wr * Dr, wr * Dr, Dr, Dr, d.field, 2);
+    g.pA = nT2; g.pB = (long long)wr * Dr * Dr; g.pC = n;
+    gemm_prec(c, d.prec, g, T2, d.R, yout);
+  }
+  if (d.prec != P64) from_prec(c, d.prec, yout, y, n * np);
+  c->matvecs++;
+}
+
+// ---------------------------------------------------------------------------
+// environment updates (bra and ket may differ in bond dimension)
+// ---------------------------------------------------------------------------
+void env_update(Ctx* c, const dmrg_env_desc& e) {
+  DMRG_REQUIRE(e.dir == 0 || e.dir == 1, "env: dir must be 0 (left) or 1 (right)");
+  DMRG_REQUIRE(e.Dl > 0 && e.Dr > 0 && e.Dlb > 0 && e.Drb > 0 && e.d > 0 && e.wl > 0 && e.wr > 0,
+               "env: bad dims");
+  DMRG_REQUIRE(!(e.w_complex && e.field == REAL), "env: complex MPO requires complex field");
+  const int Dl = e.Dl, Dr = e.Dr, Dlb = e.Dlb, Drb = e.Drb, dd = e.d, wl = e.wl, wr = e.wr;
+  const int np = e.field == CPLX ? 2 : 1;
+  const size_t es = prec_size(e.prec);
+  const bool wc = e.w_complex != 0;
+  const long long nk = (long long)Dl * dd * Dr, nb = (long long)Dlb * dd * Drb;
+  double mac = e.dir == 0
+                   ? (double)wl * Dlb * Dl * dd * Dr + (double)wl * wr * dd * dd * Dlb * Dr + (double)wr * Drb * Dlb * dd * Dr
+                   : (double)wr * Dl * dd * Dr * Drb + (double)wl * wr * dd * dd * Dl * Drb + (double)wl * Dl * dd * Drb * Dlb;
+  ProfScope ps(c, 1, (e.field == CPLX ? 8.0 : 2.0) * mac);
+  // MPS tensors in contraction precision
+  const void* ket = e.ket;
+  const void* bra = e.bra;
+  if (e.prec != P64) {
+    void* kb = ws_get(c, WS_BF16_A, nk * np * es);
+    to_prec(c, e.prec, (const double*)e.ket, kb, nk * np);
+    ket = kb;
+    if (e.bra == e.ket) {
+      bra = kb;
+    } else {
+      void* bb = ws_get(c, WS_BF16_B, nb * np * es);
+      to_prec(c, e.prec, (const double*)e.bra, bb, nb * np);
+      bra = bb;
+    }
+  }
+  // small operator
+  const int Kin = (e.dir == 0 ? wl : wr) * dd, Kout = (e.dir == 0 ? wr : wl) * dd;
+  std::vector<double> wre((size_t)Kout * Kin, 0.0), wim(wc ? (size_t)Kout * Kin : 0, 0.0);
+  const size_t nW = (size_t)wl * wr * dd * dd;
+  for (int a = 0; a < wl; ++a)
+    for (int b = 0; b < wr; ++b)
+      for (int so = 0; so < dd; ++so)
+        for (int s = 0; s < dd; ++s) {
+          size_t src = (((size_t)a * wr + b) * dd + so) * dd + s;
+          int o, i;
+          if (e.dir == 0) { o = b * dd + so; i = a * dd + s; }
+          else { o = a * dd + so; i = b * dd + s; }
+          wre[(size_t)o * Kin + i] = e.W[src];
+          if (wc) wim[(size_t)o * Kin + i] = e.W[src + nW];
+        }
+  if (e.dir == 0) {
+    // E1[(wl a')][(s b)] = L[(wl a')][a] * ket[a][(s b)]
+    const long long n1 = (long long)wl * Dlb * dd * Dr, n2 = (long long)wr * Dlb * dd * Dr;
+    void* E1 = ws_get(c, WS_ENV1, n1 * np * es);
+    void* E2 = ws_get(c, WS_ENV2, n2 * np * es);
+    GemmDesc g = gdesc(wl * Dlb, dd * Dr, Dl, Dl, dd * Dr, dd * Dr, e.field, 3);
+    g.pA = (long long)wl * Dlb * Dl; g.pB = nk; g.pC = n1;
+    gemm_prec(c, e.prec, g, e.env_in, ket, E1);
+    std::vector<int> ioff(Kin), ooff(Kout);
+    for (int a = 0; a < wl; ++a)
+      for (int s = 0; s < dd; ++s) ioff[a * dd + s] = a * Dlb * dd * Dr + s * Dr;
+    for (int b = 0; b < wr; ++b)
+      for (int s = 0; s < dd; ++s) ooff[b * dd + s] = b * Dlb * dd * Dr + s * Dr;
+    SmallOp so = upload_smallop(c, WS_WSMALL, e.prec, wre, wim, wc, Kin, Kout, ioff, ooff);
+    run_smallop(c, e.prec, e.field, so, E1, E2, Dlb, Dr, (long long)dd * Dr, (long long)dd * Dr, n1, n2);
+    // Lnew[wr][b'][b] = sum_{(a' s')} conj(bra[(a' s')][b']) E2[wr][(a' s')][b]
+    GemmDesc h = gdesc(Drb, Dr, Dlb * dd, Drb, Dr, Dr, e.field, 3);
+    h.transA = true; h.conjA = true;
+    h.batch = wr;
+    h.sA = 0; h.sB = (long long)Dlb * dd * Dr; h.sC = (long long)Drb * Dr;
+    h.pA = nb; h.pB = n2; h.pC = (long long)wr * Drb * Dr;
+    gemm_prec(c, e.prec, h, bra, E2, e.env_out);
+  } else {
+    // F1[wr][(a s)][b'] = ket[(a s)][b] * R[wr][b][b']
+    const long long n1 = (long long)wr * Dl * dd * Drb, n2 = (long long)wl * Dl * dd * Drb;
+    void* F1 = ws_get(c, WS_ENV1, n1 * np * es);
+    void* F2 = ws_get(c, WS_ENV2, n2 * np * es);
+    GemmDesc g = gdesc(Dl * dd, Drb, Dr, Dr, Drb, Drb, e.field, 3);
+    g.batch = wr;
+    g.sA = 0; g.sB = (long long)Dr * Drb; g.sC = (long long)Dl * dd * Drb;
+    g.pA = nk; g.pB = (long long)wr * Dr * Drb; g.pC = n1;
+    gemm_prec(c, e.prec, g, ket, e.env_in, F1);
+    std::vector<int> ioff(Kin), ooff(Kout);
+    for (int b = 0; b < wr; ++b)
+      for (int s = 0; s < dd; ++s) ioff[b * dd + s] = b * Dl * dd * Drb + s * Drb;
+    for (int a = 0; a < wl; ++a)
+      for (int s = 0; s < dd; ++s) ooff[a * dd + s] = a * Dl * dd * Drb + s * Drb;
+    SmallOp so = upload_smallop(c, WS_WSMALL, e.prec, wre, wim, wc, Kin, Kout, ioff, ooff);
+    run_smallop(c, e.prec, e.field, so, F1, F2, Dl, Drb, (long long)dd * Drb, (long long)dd * Drb, n1, n2);
+    // Rnew[wl][a][a'] = sum_{(s' b')} F2[wl][a][(s' b')] conj(bra[a'][(s' b')])
+    GemmDesc h = gdesc(Dl, Dlb, dd * Drb, dd * Drb, dd * Drb, Dlb, e.field, 3);
+    h.transB = true; h.conjB = true;
+    h.batch = wl;
+    h.sA = (long long)Dl * dd * Drb; h.sB = 0; h.sC = (long long)Dl * Dlb;
+    h.pA = n2; h.pB = nb; h.pC = (long long)wl * Dl * Dlb;
+    gemm_prec(c, e.prec, h, F2, bra, e.env_out);
+  }
+}
+
+}  // namespace dmrg

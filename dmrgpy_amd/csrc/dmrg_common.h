@@ -1,0 +1,128 @@
+// Shared definitions for the MI355X (gfx950) DMRG/TDVP engine.
+//
+// Conventions used by every kernel and entry point in this library:
+//  * row-major storage, 64-bit element offsets;
+//  * "field": 0 = real, 1 = complex stored as two planes (re plane, then im
+//    plane, separated by a plane stride); every complex operand is planar;
+//  * "prec":  0 = FP64 contractions, 1 = FP32 contractions (FP32 MFMA with
+//    FP32 accumulation), 2 = split-BF16x3 contractions (three BF16 MFMA
+//    products hi*hi + hi*lo + lo*hi with FP32 accumulation).  Vectors handed
+//    to the Krylov solvers, SVD inputs and outputs are always FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dmrg {
+
+enum Field { REAL = 0, CPLX = 1 };
+enum Prec { P64 = 0, P32 = 1, PB3 = 2 };
+
+// Thrown by host code inside the library; every extern "C" entry point
+// catches it and turns it into a status code plus dmrg_last_error().
+struct Error {
+  std::string msg;
+  explicit Error(std::string m) : msg(std::move(m)) {}
+};
+
+#define DMRG_CHECK_HIP(expr)                                                  \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw ::dmrg::Error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define DMRG_REQUIRE(cond, msg)                            \
+  do {                                                     \
+    if (!(cond)) throw ::dmrg::Error(std::string(msg));    \
+  } while (0)
+
+// A device buffer that only grows.  Growth synchronises the stream first so
+// the old allocation is not freed under a running kernel.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void* get(size_t need, hipStream_t s) {
+    if (need > bytes) {
+      if (ptr) {
+        DMRG_CHECK_HIP(hipStreamSynchronize(s));
+        DMRG_CHECK_HIP(hipFree(ptr));
+        ptr = nullptr;
+      }
+      size_t nb = need + need / 8 + 256;
+      DMRG_CHECK_HIP(hipMalloc(&ptr, nb));
+      bytes = nb;
+    }
+    return ptr;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+// Workspace slots.  Each slot is owned by one stage of one algorithm, so
+// nested calls never alias (a Lanczos loop's matvec uses the H_eff slots,
+// the Lanczos basis uses its own).
+enum Slot {
+  WS_T1 = 0, WS_T2, WS_XIN, WS_YOUT, WS_WSMALL, WS_ENV1, WS_ENV2,
+  WS_KRYLOV_Q, WS_KRYLOV_W, WS_KRYLOV_RED, WS_KRYLOV_COEF,
+  WS_SVD_GRAM, WS_SVD_EV, WS_SVD_TMP, WS_SVD_TMP2, WS_SVD_INFO, WS_SVD_CPLX,
+  WS_BF16_A, WS_BF16_B, WS_REFINE,
+  WS_NSLOTS
+};
+
+struct ProfEntry {
+  hipEvent_t start, stop;
+  double flops;
+  int kind;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  DevBuf ws[WS_NSLOTS];
+  double* host_scalars = nullptr;  // pinned, for scalar read-back
+  // profiling of H_eff applies (kind 0) and environment updates (kind 1)
+  bool prof_on = false;
+  std::vector<ProfEntry> prof;
+  std::vector<hipEvent_t> event_pool;
+  double prof_time_ms[4] = {0, 0, 0, 0};
+  double prof_flops[4] = {0, 0, 0, 0};
+  long long prof_calls[4] = {0, 0, 0, 0};
+  long long matvecs = 0;
+};
+
+void* ws_get(Ctx* c, Slot s, size_t bytes);
+
+// ---- GEMM (gemm.hip) ------------------------------------------------------
+// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b], row-major, logical A is
+// M x K, B is K x N.  transA: A stored K x M (element (m,k) at k*lda+m).
+// transB: B stored N x K.  Complex (field==CPLX): planar operands, conjA /
+// conjB negate the imaginary planes.  tag only distinguishes kernel names
+// in profiles (0 generic, 1 H_eff L-step, 2 H_eff R-step, 3 env, 4 svd,
+// 5 backward).
+struct GemmDesc {
+  int M, N, K;
+  int lda, ldb, ldc;
+  long long sA, sB, sC;      // batch strides (elements)
+  long long pA, pB, pC;      // plane strides (elements), complex only
+  int batch;
+  bool transA, transB;
+  bool conjA, conjB;
+  double alpha, beta;
+  int field;
+  int tag;
+};
+void gemm_f64(Ctx* c, const GemmDesc& g, const double* A, const double* B, double* C);
+void gemm_f32(Ctx* c, const GemmDesc& g, const float* A, const float* B, float* C);
+// split-BF16x3: FP32 operands split on the fly into bf16 hi/lo pairs.
+void gemm_bf16x3(Ctx* c, const GemmDesc& g, const float* A, const float* B, float* C);
+
+}  // namespace dmrg

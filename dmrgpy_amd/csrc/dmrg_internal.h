@@ -1,0 +1,66 @@
+// Internal declarations shared by the HIP translation units of the library.
+#pragma once
+
+#include <vector>
+
+#include "dmrg_common.h"
+#include "../../include/dmrg_hip.h"
+
+namespace dmrg {
+
+struct SmallOp {
+  const void* W = nullptr;     // [KoutPad][Kin] (+ imag plane), contraction precision
+  const int* ioff = nullptr;   // [Kin]
+  const int* ooff = nullptr;   // [Kout]
+  int Kin = 0, Kout = 0, KoutPad = 0;
+  bool wcplx = false;
+};
+
+struct HeffOp {
+  dmrg_heff_desc d;
+  SmallOp sop;
+  int P = 1;            // product of the physical dims acted on
+  long long n = 0;      // vector length per plane
+  double flops_per_apply = 0;
+};
+
+// Records a pair of HIP events around a region on the context stream when
+// profiling is on; kind 0 = H_eff apply, 1 = environment update, 2 = SVD,
+// 3 = Krylov vector work.
+struct ProfScope {
+  Ctx* c;
+  int kind;
+  double flops;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(Ctx* c_, int kind_, double flops_);
+  ~ProfScope();
+};
+
+size_t prec_size(int prec);
+void to_prec(Ctx* c, int prec, const double* src, void* dst, long long n);
+void from_prec(Ctx* c, int prec, const void* src, double* dst, long long n);
+SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& wre,
+                       const std::vector<double>& wim, bool wcplx, int Kin, int Kout,
+                       const std::vector<int>& ioff, const std::vector<int>& ooff);
+void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, void* out, int X,
+                 int inner, long long sx_in, long long sx_out, long long pin, long long pout);
+void gemm_prec(Ctx* c, int prec, const GemmDesc& g, const void* A, const void* B, void* C);
+GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag);
+
+double heff_flops(const dmrg_heff_desc& d);
+HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d);
+void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y);
+void env_update(Ctx* c, const dmrg_env_desc& e);
+
+// Krylov vector primitives (FP64, planar complex when field == CPLX); vectors
+// of length n per plane, a "block" of k vectors is stored with stride ldq
+// elements between consecutive vectors (each vector's planes are n apart).
+void vec_dot_block(Ctx* c, int field, const double* Q, long long ldq, int k, const double* w, long long n,
+                   double* out /* device, k (re) or 2k (re..., im...) */);
+void vec_sub_block(Ctx* c, int field, const double* Q, long long ldq, int k, const double* coef,
+                   double* w, long long n);
+void vec_combine(Ctx* c, int field, const double* Q, long long ldq, int k, const double* coef,
+                 int coef_cplx, double scale, double* x, long long n);
+double vec_dot_real(Ctx* c, int field, const double* a, const double* b, long long n);
+
+}  // namespace dmrg

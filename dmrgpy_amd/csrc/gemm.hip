@@ -1,0 +1,456 @@
+// MFMA GEMM kernels for gfx950 (CDNA4): FP64 and FP32 via the 16x16x4
+// MFMA shapes, split-BF16x3 via the 16x16x32 BF16 MFMA.  256 threads = four
+// 64-wide waves in a 2x2 arrangement; every wave owns a (16*TM) x (16*TN)
+// patch of the block tile, so a block computes (32*TM) x (32*TN).  Operand
+// tiles are staged through LDS with the next tile prefetched into registers
+// while the current one is multiplied.
+//
+// MFMA lane layout (16x16xK shapes): lane l supplies A[i = l%16][k-slot l/16]
+// and B[k-slot l/16][j = l%16]; the accumulator holds C[4*(l/16) + r][l%16],
+// r = 0..3.  The k-slots of one instruction may map to any 4 (FP32/FP64) or
+// 32 (BF16, 8 per slot) distinct k values as long as A and B agree, so each
+// lane reads 4 (resp. 8) consecutive k from LDS in one vector access.
+#include "dmrg_common.h"
+
+namespace dmrg {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef dbl4 acc_t;
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  typedef flt4 acc_t;
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+struct GemmArgs {
+  int M, N, K, lda, ldb, ldc;
+  long long sA, sB, sC, pA, pB, pC;
+  double alpha, beta;
+  int conjA, conjB;
+};
+
+// ---------------------------------------------------------------------------
+// FP64 / FP32 kernel
+// ---------------------------------------------------------------------------
+template <typename T, bool CX, int TM, int TN, bool TA, bool TB, int TAG>
+__global__ __launch_bounds__(256) void gemm_mfma(const T* __restrict__ A, const T* __restrict__ B,
+                                                 T* __restrict__ C, GemmArgs g) {
+  constexpr int BM = 32 * TM, BN = 32 * TN, BK = 16;
+  constexpr int PAD = (sizeof(T) == 8) ? 2 : 4;
+  constexpr int NP = CX ? 2 : 1;
+  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+  typedef typename Mfma<T>::acc_t acc_t;
+
+  __shared__ T As[NP][BM][BK + PAD];
+  __shared__ T Bs[NP][BN][BK + PAD];
+
+  const long long bz = blockIdx.z;
+  A += bz * g.sA;
+  B += bz * g.sB;
+  C += bz * g.sC;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int M = g.M, N = g.N, K = g.K;
+
+  acc_t acc[NP][TM][TN];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b) acc[p][a][b] = acc_t{0, 0, 0, 0};
+
+  T ra[NP][LA], rb[NP][LB];
+  const T sgnA = (CX && g.conjA) ? T(-1) : T(1);
+  const T sgnB = (CX && g.conjB) ? T(-1) : T(1);
+
+  auto load_regs = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < LA; ++e) {
+      int idx = tid + 256 * e;
+      int r, c;
+      if (!TA) { r = idx / BK; c = idx % BK; } else { c = idx / BM; r = idx % BM; }
+      int gm = m0 + r, gk = k0 + c;
+      bool ok = (gm < M) && (gk < K);
+      long long off = TA ? (long long)gk * g.lda + gm : (long long)gm * g.lda + gk;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) ra[p][e] = ok ? A[off + p * g.pA] : T(0);
+    }
+#pragma unroll
+    for (int e = 0; e < LB; ++e) {
+      int idx = tid + 256 * e;
+      int r, c;  // r: n within tile, c: k within tile
+      if (!TB) { c = idx / BN; r = idx % BN; } else { r = idx / BK; c = idx % BK; }
+      int gn = n0 + r, gk = k0 + c;
+      bool ok = (gn < N) && (gk < K);
+      long long off = TB ? (long long)gn * g.ldb + gk : (long long)gk * g.ldb + gn;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) rb[p][e] = ok ? B[off + p * g.pB] : T(0);
+    }
+  };
+  auto store_regs = [&]() {
+#pragma unroll
+    for (int e = 0; e < LA; ++e) {
+      int idx = tid + 256 * e;
+      int r, c;
+      if (!TA) { r = idx / BK; c = idx % BK; } else { c = idx / BM; r = idx % BM; }
+      As[0][r][c] = ra[0][e];
+      if (CX) As[NP - 1][r][c] = sgnA * ra[NP - 1][e];
+    }
+#pragma unroll
+    for (int e = 0; e < LB; ++e) {
+      int idx = tid + 256 * e;
+      int r, c;
+      if (!TB) { c = idx / BN; r = idx % BN; } else { r = idx / BK; c = idx % BK; }
+      Bs[0][r][c] = rb[0][e];
+      if (CX) Bs[NP - 1][r][c] = sgnB * rb[NP - 1][e];
+    }
+  };
+
+  load_regs(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    store_regs();
+    __syncthreads();
+    if (k0 + BK < K) load_regs(k0 + BK);
+    T fa[NP][TM][4], fb[NP][TN][4];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) fa[p][a][t] = As[p][wm * TM * 16 + a * 16 + li][lk * 4 + t];
+#pragma unroll
+      for (int b = 0; b < TN; ++b)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) fb[p][b][t] = Bs[p][wn * TN * 16 + b * 16 + li][lk * 4 + t];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int a = 0; a < TM; ++a) {
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+          if (!CX) {
+            acc[0][a][b] = Mfma<T>::op(fa[0][a][t], fb[0][b][t], acc[0][a][b]);
+          } else {
+            acc[0][a][b] = Mfma<T>::op(fa[0][a][t], fb[0][b][t], acc[0][a][b]);
+            acc[0][a][b] = Mfma<T>::op(-fa[NP - 1][a][t], fb[NP - 1][b][t], acc[0][a][b]);
+            acc[NP - 1][a][b] = Mfma<T>::op(fa[0][a][t], fb[NP - 1][b][t], acc[NP - 1][a][b]);
+            acc[NP - 1][a][b] = Mfma<T>::op(fa[NP - 1][a][t], fb[0][b][t], acc[NP - 1][a][b]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  const T alpha = T(g.alpha), beta = T(g.beta);
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int row = m0 + wm * TM * 16 + a * 16 + lk * 4 + r;
+        int col = n0 + wn * TN * 16 + b * 16 + li;
+        if (row < M && col < N) {
+          long long off = (long long)row * g.ldc + col;
+#pragma unroll
+          for (int p = 0; p < NP; ++p) {
+            T v = alpha * acc[p][a][b][r];
+            if (g.beta != 0.0) v += beta * C[off + p * g.pC];
+            C[off + p * g.pC] = v;
+          }
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// split-BF16x3 kernel: FP32 operands, x = hi + lo with hi = bf16(x) and
+// lo = bf16(x - hi); C += Ahi*Bhi + Ahi*Blo + Alo*Bhi with FP32 accumulation.
+// The split happens once per element when the tile is written to LDS.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+template <bool CX, int TM, int TN, bool TA, bool TB, int TAG>
+__global__ __launch_bounds__(256) void gemm_bf16x3_k(const float* __restrict__ A, const float* __restrict__ B,
+                                                     float* __restrict__ C, GemmArgs g) {
+  constexpr int BM = 32 * TM, BN = 32 * TN, BK = 32;
+  constexpr int PAD = 8;
+  constexpr int NP = CX ? 2 : 1;
+  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+
+  __shared__ __bf16 Ah[NP][BM][BK + PAD], Al[NP][BM][BK + PAD];
+  __shared__ __bf16 Bh[NP][BN][BK + PAD], Bl[NP][BN][BK + PAD];
+
+  const long long bz = blockIdx.z;
+  A += bz * g.sA;
+  B += bz * g.sB;
+  C += bz * g.sC;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int M = g.M, N = g.N, K = g.K;
+
+  flt4 acc[NP][TM][TN];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b) acc[p][a][b] = flt4{0, 0, 0, 0};
+
+  float ra[NP][LA], rb[NP][LB];
+  const float sgnA = (CX && g.conjA) ? -1.f : 1.f;
+  const float sgnB = (CX && g.conjB) ? -1.f : 1.f;
+
+  auto load_regs = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < LA; ++e) {
+      int idx = tid + 256 * e;
+      int r, c;
+      if (!TA) { r = idx / BK; c = idx % BK; } else { c = idx / BM; r = idx % BM; }
+      int gm = m0 + r, gk = k0 + c;
+      bool ok = (gm < M) && (gk < K);
+      long long off = TA ? (long long)gk * g.lda + gm : (long long)gm * g.lda + gk;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) ra[p][e] = ok ? A[off + p * g.pA] : 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < LB; ++e) {
+      int idx = tid + 256 * e;
+      int r, c;
+      if (!TB) { c = idx / BN; r = idx % BN; } else { r = idx / BK; c = idx % BK; }
+      int gn = n0 + r, gk = k0 + c;
+      bool ok = (gn < N) && (gk < K);
+      long long off = TB ? (long long)gn * g.ldb + gk : (long long)gk * g.ldb + gn;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) rb[p][e] = ok ? B[off + p * g.pB] : 0.f;
+    }
+  };
+  auto store_regs = [&]() {
+#pragma unroll
+    for (int e = 0; e < LA; ++e) {
+      int idx = tid + 256 * e;
+      int r, c;
+      if (!TA) { r = idx / BK; c = idx % BK; } else { c = idx / BM; r = idx % BM; }
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        float x = (p == 1) ? sgnA * ra[p][e] : ra[p][e];
+        __bf16 h, l;
+        split_bf16(x, h, l);
+        Ah[p][r][c] = h;
+        Al[p][r][c] = l;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < LB; ++e) {
+      int idx = tid + 256 * e;
+      int r, c;
+      if (!TB) { c = idx / BN; r = idx % BN; } else { r = idx / BK; c = idx % BK; }
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        float x = (p == 1) ? sgnB * rb[p][e] : rb[p][e];
+        __bf16 h, l;
+        split_bf16(x, h, l);
+        Bh[p][r][c] = h;
+        Bl[p][r][c] = l;
+      }
+    }
+  };
+
+  load_regs(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    store_regs();
+    __syncthreads();
+    if (k0 + BK < K) load_regs(k0 + BK);
+    bf16x8 ah[NP][TM], al[NP][TM], bh[NP][TN], bl[NP][TN];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+      for (int a = 0; a < TM; ++a) {
+        int row = wm * TM * 16 + a * 16 + li;
+        ah[p][a] = *reinterpret_cast<const bf16x8*>(&Ah[p][row][lk * 8]);
+        al[p][a] = *reinterpret_cast<const bf16x8*>(&Al[p][row][lk * 8]);
+      }
+#pragma unroll
+      for (int b = 0; b < TN; ++b) {
+        int row = wn * TN * 16 + b * 16 + li;
+        bh[p][b] = *reinterpret_cast<const bf16x8*>(&Bh[p][row][lk * 8]);
+        bl[p][b] = *reinterpret_cast<const bf16x8*>(&Bl[p][row][lk * 8]);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b) {
+        if (!CX) {
+          acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0][a], bh[0][b], acc[0][a][b], 0, 0, 0);
+          acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][a], bl[0][b], acc[0][a][b], 0, 0, 0);
+          acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][a], bh[0][b], acc[0][a][b], 0, 0, 0);
+        } else {
+          bf16x8 nah = -ah[1][a], nal = -al[1][a];
+          acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0][a], bh[0][b], acc[0][a][b], 0, 0, 0);
+          acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][a], bl[0][b], acc[0][a][b], 0, 0, 0);
+          acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][a], bh[0][b], acc[0][a][b], 0, 0, 0);
+          acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nal, bh[1][b], acc[0][a][b], 0, 0, 0);
+          acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nah, bl[1][b], acc[0][a][b], 0, 0, 0);
+          acc[0][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nah, bh[1][b], acc[0][a][b], 0, 0, 0);
+          acc[1][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0][a], bh[1][b], acc[1][a][b], 0, 0, 0);
+          acc[1][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][a], bl[1][b], acc[1][a][b], 0, 0, 0);
+          acc[1][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][a], bh[1][b], acc[1][a][b], 0, 0, 0);
+          acc[1][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[1][a], bh[0][b], acc[1][a][b], 0, 0, 0);
+          acc[1][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[1][a], bl[0][b], acc[1][a][b], 0, 0, 0);
+          acc[1][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[1][a], bh[0][b], acc[1][a][b], 0, 0, 0);
+        }
+      }
+    __syncthreads();
+  }
+
+  const float alpha = (float)g.alpha, beta = (float)g.beta;
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int row = m0 + wm * TM * 16 + a * 16 + lk * 4 + r;
+        int col = n0 + wn * TN * 16 + b * 16 + li;
+        if (row < M && col < N) {
+          long long off = (long long)row * g.ldc + col;
+#pragma unroll
+          for (int p = 0; p < NP; ++p) {
+            float v = alpha * acc[p][a][b][r];
+            if (g.beta != 0.0) v += beta * C[off + p * g.pC];
+            C[off + p * g.pC] = v;
+          }
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// host dispatch
+// ---------------------------------------------------------------------------
+static GemmArgs make_args(const GemmDesc& d) {
+  GemmArgs g;
+  g.M = d.M; g.N = d.N; g.K = d.K;
+  g.lda = d.lda; g.ldb = d.ldb; g.ldc = d.ldc;
+  g.sA = d.sA; g.sB = d.sB; g.sC = d.sC;
+  g.pA = d.pA; g.pB = d.pB; g.pC = d.pC;
+  g.alpha = d.alpha; g.beta = d.beta;
+  g.conjA = d.conjA; g.conjB = d.conjB;
+  return g;
+}
+
+static void check_desc(const GemmDesc& d) {
+  DMRG_REQUIRE(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 1, "gemm: negative size");
+  DMRG_REQUIRE(d.batch <= 65535, "gemm: batch too large");
+  DMRG_REQUIRE(d.ldc >= d.N || d.M <= 1, "gemm: ldc < N");
+  DMRG_REQUIRE(d.transA ? (d.lda >= d.M || d.K <= 1) : (d.lda >= d.K || d.M <= 1), "gemm: bad lda");
+  DMRG_REQUIRE(d.transB ? (d.ldb >= d.K || d.N <= 1) : (d.ldb >= d.N || d.K <= 1), "gemm: bad ldb");
+}
+
+// Launch helper: picks the instantiation for (transA, transB, tag).
+#define DMRG_GEMM_TAGS(X) X(0) X(1) X(2) X(3) X(4) X(5)
+
+template <typename T, bool CX, int TM, int TN, bool TA, bool TB>
+static void launch_tagged(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
+  GemmArgs g = make_args(d);
+  dim3 grid((d.N + 32 * TN - 1) / (32 * TN), (d.M + 32 * TM - 1) / (32 * TM), d.batch);
+  DMRG_REQUIRE(grid.y <= 65535, "gemm: M too large for one launch");
+  switch (d.tag) {
+#define CASE(t) \
+  case t: gemm_mfma<T, CX, TM, TN, TA, TB, t><<<grid, 256, 0, c->stream>>>(A, B, C, g); break;
+    DMRG_GEMM_TAGS(CASE)
+#undef CASE
+    default: throw Error("gemm: bad tag");
+  }
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
+template <typename T, bool CX, int TM, int TN>
+static void launch_tt(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
+  if (!d.transA && !d.transB) launch_tagged<T, CX, TM, TN, false, false>(c, d, A, B, C);
+  else if (d.transA && !d.transB) launch_tagged<T, CX, TM, TN, true, false>(c, d, A, B, C);
+  else if (!d.transA && d.transB) launch_tagged<T, CX, TM, TN, false, true>(c, d, A, B, C);
+  else launch_tagged<T, CX, TM, TN, true, true>(c, d, A, B, C);
+}
+
+// Tile choice: large problems use 128x128 (real) / 64x64 (complex) tiles;
+// problems that would leave the 256 CUs mostly idle drop to 64x64 / 32x32.
+template <typename T>
+static void gemm_any(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
+  check_desc(d);
+  if (d.M == 0 || d.N == 0) return;
+  long long tiles128 = (long long)((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch;
+  if (d.field == CPLX) {
+    long long tiles64 = (long long)((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch;
+    if (tiles64 >= 512) launch_tt<T, true, 2, 2>(c, d, A, B, C);
+    else launch_tt<T, true, 1, 1>(c, d, A, B, C);
+  } else {
+    if (tiles128 >= 256) launch_tt<T, false, 4, 4>(c, d, A, B, C);
+    else launch_tt<T, false, 2, 2>(c, d, A, B, C);
+  }
+}
+
+void gemm_f64(Ctx* c, const GemmDesc& d, const double* A, const double* B, double* C) {
+  gemm_any<double>(c, d, A, B, C);
+}
+void gemm_f32(Ctx* c, const GemmDesc& d, const float* A, const float* B, float* C) {
+  gemm_any<float>(c, d, A, B, C);
+}
+
+template <bool CX, int TM, int TN, bool TA, bool TB>
+static void launch_b3_tagged(Ctx* c, const GemmDesc& d, const float* A, const float* B, float* C) {
+  GemmArgs g = make_args(d);
+  dim3 grid((d.N + 32 * TN - 1) / (32 * TN), (d.M + 32 * TM - 1) / (32 * TM), d.batch);
+  DMRG_REQUIRE(grid.y <= 65535, "gemm: M too large for one launch");
+  switch (d.tag) {
+#define CASE(t) \
+  case t: gemm_bf16x3_k<CX, TM, TN, TA, TB, t><<<grid, 256, 0, c->stream>>>(A, B, C, g); break;
+    DMRG_GEMM_TAGS(CASE)
+#undef CASE
+    default: throw Error("gemm: bad tag");
+  }
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
+template <bool CX, int TM, int TN>
+static void launch_b3_tt(Ctx* c, const GemmDesc& d, const float* A, const float* B, float* C) {
+  if (!d.transA && !d.transB) launch_b3_tagged<CX, TM, TN, false, false>(c, d, A, B, C);
+  else if (d.transA && !d.transB) launch_b3_tagged<CX, TM, TN, true, false>(c, d, A, B, C);
+  else if (!d.transA && d.transB) launch_b3_tagged<CX, TM, TN, false, true>(c, d, A, B, C);
+  else launch_b3_tagged<CX, TM, TN, true, true>(c, d, A, B, C);
+}
+
+void gemm_bf16x3(Ctx* c, const GemmDesc& d, const float* A, const float* B, float* C) {
+  check_desc(d);
+  if (d.M == 0 || d.N == 0) return;
+  long long tiles128 = (long long)((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch;
+  if (d.field == CPLX) {
+    long long tiles64 = (long long)((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch;
+    if (tiles64 >= 512) launch_b3_tt<true, 2, 2>(c, d, A, B, C);
+    else launch_b3_tt<true, 1, 1>(c, d, A, B, C);
+  } else {
+    if (tiles128 >= 256) launch_b3_tt<false, 4, 4>(c, d, A, B, C);
+    else launch_b3_tt<false, 2, 2>(c, d, A, B, C);
+  }
+}
+
+}  // namespace dmrg

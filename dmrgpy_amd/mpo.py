@@ -1,0 +1,224 @@
+"""Terms -> MPO for the engine (host side, numpy).
+
+Input is the session wire format ``[(coef, [(name, site_1based), ...]), ...]``
+(``src/dmrgpy/multioperator.py:158-184``).  Each term is first resolved into
+one matrix per site with the reference's term rules (``autompo.py:36-103``:
+factor sort with fermion signs for 'C'-names, F strings threaded between
+fermionic factors, same-site factors multiplied in order).  The MPO is then
+built directly as a finite-state automaton: bond states are "nothing placed
+yet", "term finished" and one state per distinct operator prefix, so terms
+that share a prefix (e.g. every S_i S_j with the same i) share a state and a
+nearest-neighbour Heisenberg chain gets w = 5, a Hubbard chain w = 6.  The
+coupling of a term sits in exactly one MPO element (the last site of the
+term), which makes dE/dJ a plain MPO-element derivative.
+
+Realification: a resolved local matrix that is purely imaginary is replaced
+by its imaginary part and the factor i moved into the coefficient (Sy ->
+-i ISy), so Hamiltonians such as SxSx+SySy+SzSz become real MPOs and run on
+the real FP32 / FP64 kernels.  The dense MPO keeps complex storage only when
+some element stays complex.
+
+Output tensors: ``W[np, wl, wr, s_out, s_in]`` float64 (np = 1 real, 2
+complex planes), wl = 1 at the first site and wr = 1 at the last.
+"""
+
+import numpy as np
+
+from . import sites as _sites
+
+
+class Term:
+    __slots__ = ("coef", "mats", "first", "last")
+
+    def __init__(self, coef, mats, first, last):
+        self.coef = coef
+        self.mats = mats        # list (per site) of matrix ids, -1 = identity
+        self.first = first      # 0-based first non-identity site
+        self.last = last        # 0-based last non-identity site
+
+
+class MatrixTable:
+    """Interns local matrices so automaton states can be keyed by ids."""
+
+    def __init__(self):
+        self._ids = {}
+        self.mats = []
+
+    def intern(self, m):
+        m = np.asarray(m, dtype=complex)
+        key = (m.shape, np.round(m, 14).tobytes())
+        i = self._ids.get(key)
+        if i is None:
+            i = len(self.mats)
+            self._ids[key] = i
+            self.mats.append(m)
+        return i
+
+
+def resolve(codes, coef, factors):
+    """(coef, [standard-convention matrix per site]) -- the reference's
+    HTerm.add + resolve (autompo.py:47-103)."""
+    ops = []
+    c = complex(coef)
+    for name, site in factors:
+        site = int(site)
+        if site < 1 or site > len(codes):
+            raise ValueError("term site {} out of range 1..{}".format(site, len(codes)))
+        idx = 0
+        while idx < len(ops) and ops[idx][1] <= site:
+            idx += 1
+        if idx < len(ops) and _sites.is_fermionic(name):
+            if sum(1 for nm, _ in ops[idx:] if _sites.is_fermionic(nm)) % 2 == 1:
+                c = -c
+        ops.insert(idx, (name, site))
+    by_site = {}
+    for name, site in ops:
+        by_site.setdefault(site, []).append(name)
+    mats = []
+    carry = False
+    for site in range(1, len(codes) + 1):
+        code = codes[site - 1]
+        names = by_site.get(site)
+        if names is None:
+            mats.append(_sites.matrix(code, "F") if carry else None)
+            continue
+        odd = sum(1 for nm in names if _sites.is_fermionic(nm)) % 2 == 1
+        ms = [_sites.matrix(code, nm) for nm in names]
+        if carry != odd:
+            ms.append(_sites.matrix(code, "F"))
+        comb = ms[0]
+        for mm in ms[1:]:
+            comb = comb @ mm
+        mats.append(comb)
+        carry = carry != odd
+    return c, mats
+
+
+def _realify(c, mats):
+    """Move factors of i out of purely imaginary local matrices."""
+    out = []
+    for m in mats:
+        if m is None:
+            out.append(None)
+            continue
+        if np.abs(m.real).max() == 0.0 and np.abs(m.imag).max() > 0.0:
+            out.append(m.imag.astype(complex))
+            c = c * 1j
+        else:
+            out.append(m)
+    return c, out
+
+
+def _is_identity(m):
+    return m is None or (m.shape[0] == m.shape[1] and np.array_equal(m, np.eye(m.shape[0])))
+
+
+def compile_terms(codes, terms, table):
+    out = []
+    for coef, factors in terms:
+        c, mats = resolve(codes, coef, [(str(n), int(s)) for n, s in factors])
+        c, mats = _realify(c, mats)
+        if c == 0:
+            continue
+        ids = [-1 if _is_identity(m) else table.intern(m) for m in mats]
+        nontriv = [i for i, k in enumerate(ids) if k >= 0]
+        if not nontriv:              # a multiple of the identity: one-site term on site 1
+            ids[0] = table.intern(np.eye(_sites.dim(codes[0]), dtype=complex))
+            nontriv = [0]
+        out.append(Term(c, ids, nontriv[0], nontriv[-1]))
+    return out
+
+
+class MPO:
+    """Host MPO in the engine layout plus bookkeeping for gradients."""
+
+    def __init__(self, W, codes, coupling_sites=None):
+        self.W = W                       # list of [np, wl, wr, d, d] float64
+        self.codes = list(codes)
+        self.complex = W[0].shape[0] == 2
+        self.coupling_sites = coupling_sites or []
+
+    def __len__(self):
+        return len(self.W)
+
+    def bond_dims(self):
+        return [w.shape[2] for w in self.W[:-1]]
+
+    def astype_complex(self):
+        if self.complex:
+            return self
+        W = [np.concatenate([w, np.zeros_like(w)], axis=0) for w in self.W]
+        return MPO(W, self.codes, self.coupling_sites)
+
+    def to_dense(self):
+        """Dense standard-convention matrix (small chains only)."""
+        def cw(w):
+            return w[0] + 1j * w[1] if w.shape[0] == 2 else w[0].astype(complex)
+        T = cw(self.W[0])[0]                          # [wr, out, in]
+        T = T.transpose(1, 2, 0)                      # [O, I, w]
+        for w in self.W[1:]:
+            Wc = cw(w)                                # [wl, wr, o, i]
+            T = np.einsum("OIa,abji->OjIib", T, Wc)
+            O, j, I, i, b = T.shape
+            T = T.reshape(O * j, I * i, b)
+        return T[:, :, 0]
+
+
+def build_mpo(codes, terms, force_complex=False):
+    """Finite-state-automaton MPO of the given terms (see module docstring)."""
+    n = len(codes)
+    dims = [_sites.dim(c) for c in codes]
+    table = MatrixTable()
+    tl = compile_terms(codes, terms, table)
+    START, DONE = 0, 1
+    # bond b sits between site b-1 and site b (0-based sites); bonds 1..n-1
+    states = [dict() for _ in range(n + 1)]       # prefix key -> index
+    nstate = [1] + [2] * (n - 1) + [1]
+
+    def key_of(t, b):
+        return tuple((s, t.mats[s]) for s in range(t.first, b))
+
+    for t in tl:
+        for b in range(t.first + 1, t.last + 1):
+            k = key_of(t, b)
+            if k not in states[b]:
+                states[b][k] = nstate[b]
+                nstate[b] += 1
+    cplx = force_complex or any(abs(t.coef.imag) > 0 for t in tl) or \
+        any(np.abs(m.imag).max() > 0 for m in table.mats)
+    npl = 2 if cplx else 1
+    W = []
+    for s in range(n):
+        wl = nstate[s] if s > 0 else 1
+        wr = nstate[s + 1] if s < n - 1 else 1
+        d = dims[s]
+        Wc = np.zeros((wl, wr, d, d), dtype=complex)
+        eye = np.eye(d)
+        # identity channels (start->start before the term region, done->done after)
+        if s > 0 and s < n - 1:
+            Wc[START, START] += eye
+            Wc[DONE, DONE] += eye
+        elif s == 0 and n > 1:
+            Wc[0, START] += eye
+        elif s == n - 1 and n > 1:
+            Wc[DONE, 0] += eye
+        for t in tl:
+            if s < t.first or s > t.last:
+                continue
+            mid = t.mats[s]
+            m = eye if mid < 0 else table.mats[mid]
+            if s == t.first:
+                fr = 0 if s == 0 else START
+            else:
+                fr = states[s][key_of(t, s)]
+            if s == t.last:
+                to = 0 if s == n - 1 else DONE
+                Wc[fr, to] += t.coef * m
+            else:
+                to = states[s + 1][key_of(t, s + 1)]
+                Wc[fr, to] = m
+        if n == 1:
+            Wc = Wc[:1, :1]
+        planes = [Wc.real] + ([Wc.imag] if cplx else [])
+        W.append(np.ascontiguousarray(np.stack(planes), dtype=np.float64))
+    return MPO(W, codes)

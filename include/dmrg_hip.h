@@ -1,0 +1,143 @@
+/* C ABI of the MI355X (gfx950) two-site DMRG / TDVP engine, libdmrg_hip.so.
+ *
+ * Every entry point takes plain pointers and sizes, runs on the HIP stream
+ * the context was created with, and returns 0 on success or a negative
+ * status; dmrg_last_error() then holds a message.  Nothing here aborts the
+ * process.  Device pointers are HIP device memory; "host" pointers are
+ * ordinary host memory read during the call.
+ *
+ * Storage conventions (row-major):
+ *   field 0 = real, 1 = complex stored as two planes (real plane then
+ *   imaginary plane, each of the tensor's full size);
+ *   prec  0 = FP64 contractions, 1 = FP32 MFMA contractions, 2 = split-BF16x3
+ *   MFMA contractions (FP32 accumulation); Krylov vectors, MPS tensors and
+ *   SVD operands are always FP64, environments are stored in the
+ *   contraction precision (double for prec 0, float otherwise).
+ *   MPS site A[Dl][d][Dr]; MPO site W[wl][wr][s_out][s_in];
+ *   left environment L[w][D_bra][D_ket]; right environment R[w][D_ket][D_bra];
+ *   two-site vector theta[Dl][d1][d2][Dr]; one-site vector C[Dl][d][Dr].
+ *
+ * The reference computes all of this in the pure-Python backend
+ * src/dmrgpy/pyitensor/ (complex128 NumPy); each entry point names the
+ * reference function it replaces.
+ */
+#ifndef DMRG_HIP_H
+#define DMRG_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dmrg_ctx dmrg_ctx;
+
+/* Effective Hamiltonian of one bond (nsite 2) or one site (nsite 1). */
+typedef struct dmrg_heff_desc {
+  int nsite;           /* 1 or 2 */
+  int field;           /* 0 real, 1 complex (planar) */
+  int prec;            /* 0 FP64, 1 FP32, 2 BF16x3 */
+  int Dl, Dr;          /* outer link dims of the vector */
+  int d1, d2;          /* physical dims (d2 unused for nsite 1) */
+  int wl, wm, wr;      /* MPO link dims (wm unused for nsite 1) */
+  const void* L;       /* device, [wl][Dl][Dl] (bra, ket), contraction precision */
+  const void* R;       /* device, [wr][Dr][Dr] (ket, bra), contraction precision */
+  const double* W1;    /* host, nsite 2: [wl][wm][d1][d1]; nsite 1: [wl][wr][d1][d1] */
+  const double* W2;    /* host, [wm][wr][d2][d2] (nsite 2 only) */
+  int w_complex;       /* W1/W2 carry an imaginary plane after the real one */
+} dmrg_heff_desc;
+
+/* One environment extension <bra|W|ket> by one site. */
+typedef struct dmrg_env_desc {
+  int dir;             /* 0: left env grows rightward, 1: right env grows leftward */
+  int field, prec;
+  int Dl, Dr;          /* ket site link dims */
+  int Dlb, Drb;        /* bra site link dims */
+  int d;               /* physical dim */
+  int wl, wr;          /* MPO link dims of the site */
+  const double* ket;   /* device FP64 [Dl][d][Dr] */
+  const double* bra;   /* device FP64 [Dlb][d][Drb]; may alias ket */
+  const double* W;     /* host FP64 [wl][wr][d][d] (+ imaginary plane) */
+  int w_complex;
+  const void* env_in;  /* device: dir 0 [wl][Dlb][Dl]; dir 1 [wr][Dr][Drb] */
+  void* env_out;       /* device: dir 0 [wr][Drb][Dr]; dir 1 [wl][Dl][Dlb] */
+} dmrg_env_desc;
+
+/* Library version (major*100 + minor). */
+int dmrg_version(void);
+/* Message of the last failed call on this thread. */
+const char* dmrg_last_error(void);
+
+/* Create a context on `device` issuing work on `stream` (a hipStream_t, may
+ * be NULL for the null stream).  The context owns its workspace. */
+int dmrg_ctx_create(int device, void* stream, dmrg_ctx** out);
+int dmrg_ctx_destroy(dmrg_ctx* ctx);
+/* Wait for all work issued through the context. */
+int dmrg_sync(dmrg_ctx* ctx);
+
+/* y = H_eff x.  Replaces the matvec built by two_site_heff / one_site_heff
+ * (src/dmrgpy/pyitensor/dmrg.py:224-275) through kernels.make_matvec
+ * (src/dmrgpy/pyitensor/kernels.py:298-445).  x, y: device FP64, length
+ * Dl*d1*d2*Dr (nsite 2) or Dl*d1*Dr (nsite 1) per plane. */
+int dmrg_heff_apply(dmrg_ctx* ctx, const dmrg_heff_desc* op, const double* x, double* y);
+
+/* <x|H_eff|x> / <x|x> in the operator's precision. */
+int dmrg_rayleigh(dmrg_ctx* ctx, const dmrg_heff_desc* op, const double* x, double* value);
+
+/* Environment extension.  Replaces _extend_left / _extend_right
+ * (src/dmrgpy/pyitensor/dmrg.py:170-201) and the per-site step of inner()
+ * (src/dmrgpy/pyitensor/mpsalgebra.py:66-106). */
+int dmrg_env_update(dmrg_ctx* ctx, const dmrg_env_desc* env);
+
+/* Lowest eigenpair of H_eff by Lanczos with full re-orthogonalisation,
+ * warm-started from x (overwritten with the normalised eigenvector); stops
+ * when the lowest Ritz value moves by < tol*max(1,|E|) or after
+ * min(maxiter, dim) steps; dims <= 3 are diagonalised densely.  res_tol > 0
+ * adds a second stop, Ritz residual <= res_tol*max(1,|E|), for the
+ * reduced-precision operators (0 keeps the reference rule alone).  Replaces
+ * _lanczos_ground_state / _local_ground_state
+ * (src/dmrgpy/pyitensor/dmrg.py:95-142, 302-333). */
+int dmrg_lanczos_ground(dmrg_ctx* ctx, const dmrg_heff_desc* op, double* x, int maxiter, double tol,
+                        double res_tol, double* energy, int* iters);
+
+/* x <- expm((cre + i cim) * H_eff) x by a Lanczos basis with full
+ * re-orthogonalisation and Saad's a-posteriori stop
+ * beta0*beta_k*|e_k^T exp(c T_k) e_1| < errgoal.  Replaces
+ * _lanczos_expm_multiply (src/dmrgpy/pyitensor/tdvp.py:82-172). */
+int dmrg_krylov_expm(dmrg_ctx* ctx, const dmrg_heff_desc* op, double* x, double cre, double cim,
+                     int maxiter, double tol, double errgoal, int* iters);
+
+/* Truncated SVD of the m x n FP64 matrix M (planar if field 1): keeps the
+ * count given by the cutoff / maxdim / mindim rule of _truncate
+ * (src/dmrgpy/pyitensor/svd.py:32-68), Gram route with the exact-SVD
+ * fallback of _svd_truncated (svd.py:108-151).  absorb 0: U left-orthonormal,
+ * V = S*Vh (a right-moving step); absorb 1: U*S, V = Vh (left-moving);
+ * absorb 2: U, Vh with S only returned.  U is written [m][keep], V
+ * [keep][n] (planar), both dense; kcap bounds keep (buffers must hold
+ * m*kcap and kcap*n elements per plane).  svals_host (length min(m,n), may
+ * be NULL) receives all singular values, descending.  Replaces svd()
+ * (svd.py:154-195) as called by _apply_local_update
+ * (src/dmrgpy/pyitensor/dmrg.py:336-352). */
+int dmrg_svd_split(dmrg_ctx* ctx, int field, const double* M, int m, int n, double cutoff, int maxdim,
+                   int mindim, int absorb, double* U, double* V, int kcap, int* keep,
+                   double* discarded, double* svals_host);
+
+/* General batched GEMM in a given precision (used by the gradient
+ * contractions and by the tests): C = alpha op(A) op(B) + beta C with the
+ * conventions of the engine's kernels; A, B, C are FP64 device buffers for
+ * prec 0 and FP32 otherwise. */
+int dmrg_gemm(dmrg_ctx* ctx, int prec, int field, int transA, int transB, int conjA, int conjB, int M,
+              int N, int K, double alpha, const void* A, int lda, long long strideA, long long planeA,
+              const void* B, int ldb, long long strideB, long long planeB, double beta, void* C, int ldc,
+              long long strideC, long long planeC, int batch);
+
+/* Profiling of H_eff applies (slot 0), environment updates (1), SVDs (2):
+ * HIP events around each call on the context stream. */
+int dmrg_prof_enable(dmrg_ctx* ctx, int on);
+int dmrg_prof_read(dmrg_ctx* ctx, double* time_ms, double* flops, long long* calls);
+/* Total H_eff applies issued through the context. */
+long long dmrg_matvec_count(dmrg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMRG_HIP_H */

@@ -1,0 +1,108 @@
+"""Host-side MPO builder of the backend (dmrgpy_amd/mpo.py) against the
+oracle's dense Kronecker Hamiltonian (the reference's AutoMPO.dense_matrix
+semantics, src/dmrgpy/pyitensor/autompo.py:144-164)."""
+
+import numpy as np
+import pytest
+
+from dmrgpy_amd import mpo as pmpo
+from dmrgpy_amd import sites as psites
+from oracle import ed
+from oracle import mpo as ompo
+from oracle import sites as osites
+
+
+def hubbard_terms(n, t=1.0, U=4.0):
+    """Native spinful chain after Jordan-Wigner (front-end form)."""
+    from dmrgpy_amd.multioperator import hubbard_terms as ht
+    return ht(n, t, U)
+
+
+CASES = {
+    "heis_s12": ([2] * 6, ed.heisenberg_terms(6)),
+    "heis_s1": ([3] * 4, ed.heisenberg_terms(4)),
+    "heis_s32": ([4] * 3, ed.heisenberg_terms(3)),
+    "xxz_random": ([2] * 5, ed.heisenberg_terms(5, np.linspace(0.6, 1.4, 4), np.linspace(1.2, 0.3, 4))),
+    "fields": ([2] * 4, [(0.3, [("Sx", 1)]), (-0.7, [("Sz", 4)]), (0.2, [("Sy", 2)]),
+                         (1.0, [("Sz", 1), ("Sz", 4)]), (0.5, [("Sp", 2), ("Sm", 3)]),
+                         (0.5, [("Sm", 2), ("Sp", 3)])]),
+    "spinless": ([0] * 5, [(-1.0, [("Adag", i), ("A", i + 1)]) for i in range(1, 5)] +
+                 [(-1.0, [("Adag", i + 1), ("A", i)]) for i in range(1, 5)] +
+                 [(0.5, [("N", i), ("N", i + 1)]) for i in range(1, 5)]),
+    "cnames": ([0] * 4, [(-1.0, [("Cdag", 1), ("C", 3)]), (-1.0, [("Cdag", 3), ("C", 1)]),
+                         (0.3, [("C", 2), ("Cdag", 4)]), (0.3, [("C", 4), ("Cdag", 2)])]),
+    "identity_shift": ([2] * 3, ed.heisenberg_terms(3) + [(-2.5, [("Id", 1)])]),
+    "complex_coupling": ([2] * 3, [(1j, [("Sp", 1), ("Sm", 2)]), (-1j, [("Sm", 1), ("Sp", 2)])]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_build_mpo_dense(name):
+    codes, terms = CASES[name]
+    m = pmpo.build_mpo(codes, terms)
+    H = ompo.dense_matrix(codes, terms)
+    assert np.abs(m.to_dense() - H).max() < 1e-13
+
+
+def test_heisenberg_is_real_w5():
+    m = pmpo.build_mpo([2] * 10, ed.heisenberg_terms(10))
+    assert not m.complex
+    assert max(m.bond_dims()) == 5
+
+
+def test_hubbard_is_real_w6():
+    codes = [1] * 4
+    terms = hubbard_terms(4)
+    m = pmpo.build_mpo(codes, terms)
+    assert not m.complex
+    assert max(m.bond_dims()) == 6
+    assert np.abs(m.to_dense() - ompo.dense_matrix(codes, terms)).max() < 1e-13
+
+
+def test_complex_mpo_kept_complex():
+    codes, terms = CASES["complex_coupling"]
+    assert pmpo.build_mpo(codes, terms).complex
+
+
+@pytest.mark.parametrize("code", [0, 1, 2, 3, 4, 5, 6])
+def test_site_tables_match_oracle(code):
+    # two independent restatements of the reference site tables agree
+    d = psites.dim(code)
+    assert d == osites.dim(code)
+    names = ["Sz", "Sx", "Sy", "Sp", "Sm"] if code >= 2 else (
+        ["A", "Adag", "N", "F"] if code == 0 else
+        ["Aup", "Adn", "Adagup", "Adagdn", "Cdn", "Cdagdn", "Nup", "Ndn", "F", "Fup", "Sz", "Sp", "Sm"])
+    for nm in names:
+        assert np.array_equal(psites.matrix(code, nm), osites.op(code, nm)), nm
+
+
+@pytest.mark.parametrize("code", [2, 3, 4, 5, 6])
+def test_spin_algebra(code):
+    S = (code - 1) / 2.0
+    sx, sy, sz = (psites.matrix(code, k) for k in ("Sx", "Sy", "Sz"))
+    assert np.abs(sx @ sy - sy @ sx - 1j * sz).max() < 1e-12
+    assert np.abs(sx @ sx + sy @ sy + sz @ sz - S * (S + 1) * np.eye(psites.dim(code))).max() < 1e-12
+
+
+def _golden(name):
+    import json
+    import os
+    cases = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "dmrg_golden.json")))
+    c = next(c for c in cases if c["name"] == name)
+    return c["codes"], [(complex(*t[0]), [tuple(f) for f in t[1]]) for t in c["terms"]]
+
+
+def test_front_end_jordan_wigner_matches_reference_terms():
+    """The front end's JW transform gives the same operator as the terms the
+    reference's own MultiOperator.to_terms produced (golden fixture)."""
+    from dmrgpy_amd.multioperator import hubbard_terms as ht, obj2MO
+    codes, ref_terms = _golden("hubbard_n6_m64_s6")
+    diff = ed.sparse_hamiltonian(codes, ht(6, 1.0, 4.0)) - ed.sparse_hamiltonian(codes, ref_terms)
+    assert abs(diff).max() < 1e-13
+    codes, ref_terms = _golden("spinless_n10_m32_s5")
+    H = 0
+    for i in range(9):
+        H = H + (-1.0) * (obj2MO([["Cdag", i], ["C", i + 1]]) + obj2MO([["Cdag", i + 1], ["C", i]]))
+        H = H + 0.5 * obj2MO([["N", i], ["N", i + 1]])
+    diff = ed.sparse_hamiltonian(codes, H.to_terms()) - ed.sparse_hamiltonian(codes, ref_terms)
+    assert abs(diff).max() < 1e-13
