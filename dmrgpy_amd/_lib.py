@@ -50,6 +50,8 @@ SIGNATURES = {
                                  c_int, c_double, c_double, P_int]),
     "dmrg_svd_split": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_int, c_int, c_int,
                                c_void_p, c_void_p, c_int, P_int, P_double, P_double]),
+    "dmrg_mpo_grad": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                              c_int, c_int, c_int, c_int, c_int, c_void_p]),
     "dmrg_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                           c_double, c_void_p, c_int, c_ll, c_ll, c_void_p, c_int, c_ll, c_ll,
                           c_double, c_void_p, c_int, c_ll, c_ll, c_int]),

@@ -6,8 +6,9 @@
 // while the current one is multiplied.
 //
 // MFMA lane layout (16x16xK shapes): lane l supplies A[i = l%16][k-slot l/16]
-// and B[k-slot l/16][j = l%16]; the accumulator holds C[4*(l/16) + r][l%16],
-// r = 0..3.  The k-slots of one instruction may map to any 4 (FP32/FP64) or
+// and B[k-slot l/16][j = l%16]; the accumulator element r (0..3) holds
+// C[4*(l/16) + r][l%16] for the FP32 and BF16 shapes and C[4*r + l/16][l%16]
+// for the FP64 shape.  The k-slots of one instruction may map to any 4 (FP32/FP64) or
 // 32 (BF16, 8 per slot) distinct k values as long as A and B agree, so each
 // lane reads 4 (resp. 8) consecutive k from LDS in one vector access.
 #include "dmrg_common.h"
@@ -163,7 +164,10 @@ __global__ __launch_bounds__(256) void gemm_mfma(const T* __restrict__ A, const 
     for (int b = 0; b < TN; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int row = m0 + wm * TM * 16 + a * 16 + lk * 4 + r;
+        // accumulator row of element r: FP32 16x16x4 holds rows 4*(l/16)+r,
+        // FP64 16x16x4 holds rows 4*r+(l/16) (measured lane layouts)
+        const int rr = (sizeof(T) == 8) ? (4 * r + lk) : (lk * 4 + r);
+        int row = m0 + wm * TM * 16 + a * 16 + rr;
         int col = n0 + wn * TN * 16 + b * 16 + li;
         if (row < M && col < N) {
           long long off = (long long)row * g.ldc + col;

@@ -155,6 +155,16 @@ class Engine:
         V = Vb[:npl * k * n].view(npl, k, n).clone()
         return U, V, k, disc.value, sv
 
+    def mpo_grad(self, field, prec, L, bra, ket, R, wl, wr):
+        """dE/dW of one site: host [np, wl, wr, d, d] float64."""
+        _, Dl, d, Dr = ket.shape
+        _, Dlb, _, Drb = bra.shape
+        npl = 2 if field == 1 else 1
+        G = np.zeros((npl, wl, wr, d, d))
+        check(self.lib.dmrg_mpo_grad(self.ctx, field, prec, _ptr(L), _ptr(bra), _ptr(ket), _ptr(R), Dl, Dr, Dlb,
+                                     Drb, d, wl, wr, _hptr(G)), "dmrg_mpo_grad")
+        return G
+
     def gemm(self, prec, field, A, B, C, M, N, K, transA=False, transB=False, conjA=False, conjB=False,
              alpha=1.0, beta=0.0, lda=None, ldb=None, ldc=None, strideA=0, strideB=0, strideC=0,
              planeA=0, planeB=0, planeC=0, batch=1):

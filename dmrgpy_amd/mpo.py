@@ -28,13 +28,15 @@ from . import sites as _sites
 
 
 class Term:
-    __slots__ = ("coef", "mats", "first", "last")
+    __slots__ = ("coef", "mats", "first", "last", "index", "source")
 
-    def __init__(self, coef, mats, first, last):
+    def __init__(self, coef, mats, first, last, index=0, source=1.0):
         self.coef = coef
         self.mats = mats        # list (per site) of matrix ids, -1 = identity
         self.first = first      # 0-based first non-identity site
         self.last = last        # 0-based last non-identity site
+        self.index = index      # position in the input term list
+        self.source = source    # coef = source * (input coefficient)
 
 
 class MatrixTable:
@@ -115,17 +117,17 @@ def _is_identity(m):
 
 def compile_terms(codes, terms, table):
     out = []
-    for coef, factors in terms:
-        c, mats = resolve(codes, coef, [(str(n), int(s)) for n, s in factors])
+    for index, (coef, factors) in enumerate(terms):
+        c, mats = resolve(codes, 1.0, [(str(n), int(s)) for n, s in factors])
         c, mats = _realify(c, mats)
-        if c == 0:
+        if complex(coef) == 0:
             continue
         ids = [-1 if _is_identity(m) else table.intern(m) for m in mats]
         nontriv = [i for i, k in enumerate(ids) if k >= 0]
         if not nontriv:              # a multiple of the identity: one-site term on site 1
             ids[0] = table.intern(np.eye(_sites.dim(codes[0]), dtype=complex))
             nontriv = [0]
-        out.append(Term(c, ids, nontriv[0], nontriv[-1]))
+        out.append(Term(c * complex(coef), ids, nontriv[0], nontriv[-1], index, c))
     return out
 
 
@@ -188,6 +190,9 @@ def build_mpo(codes, terms, force_complex=False):
         any(np.abs(m.imag).max() > 0 for m in table.mats)
     npl = 2 if cplx else 1
     W = []
+    # (input term index, site, wl state, wr state, matrix, source factor):
+    # where each coefficient sits; the element equals source * coef * matrix
+    coupling_sites = []
     for s in range(n):
         wl = nstate[s] if s > 0 else 1
         wr = nstate[s + 1] if s < n - 1 else 1
@@ -214,6 +219,7 @@ def build_mpo(codes, terms, force_complex=False):
             if s == t.last:
                 to = 0 if s == n - 1 else DONE
                 Wc[fr, to] += t.coef * m
+                coupling_sites.append((t.index, s, fr, to, m, t.source))
             else:
                 to = states[s + 1][key_of(t, s + 1)]
                 Wc[fr, to] = m
@@ -221,4 +227,4 @@ def build_mpo(codes, terms, force_complex=False):
             Wc = Wc[:1, :1]
         planes = [Wc.real] + ([Wc.imag] if cplx else [])
         W.append(np.ascontiguousarray(np.stack(planes), dtype=np.float64))
-    return MPO(W, codes)
+    return MPO(W, codes, coupling_sites)

@@ -120,6 +120,15 @@ int dmrg_svd_split(dmrg_ctx* ctx, int field, const double* M, int m, int n, doub
                    int mindim, int absorb, double* U, double* V, int kcap, int* keep,
                    double* discarded, double* svals_host);
 
+/* Gradient of <bra|H|ket> with respect to one MPO site tensor W[wl][wr][s'][s]:
+ * G = sum L[wl][a'][a] conj(bra[a'][s'][b']) ket[a][s][b] R[wr][b][b'], written
+ * to host memory as G[np][wl][wr][d][d] (FP64).  L [wl][Dlb][Dl] and
+ * R [wr][Dr][Drb] are environments in the contraction precision.  Has no
+ * counterpart in the reference (which has no gradients); with a normalised
+ * ground state it gives dE/dW (Hellmann-Feynman) for every coupling. */
+int dmrg_mpo_grad(dmrg_ctx* ctx, int field, int prec, const void* L, const double* bra, const double* ket,
+                  const void* R, int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, double* G_host);
+
 /* General batched GEMM in a given precision (used by the gradient
  * contractions and by the tests): C = alpha op(A) op(B) + beta C with the
  * conventions of the engine's kernels; A, B, C are FP64 device buffers for
