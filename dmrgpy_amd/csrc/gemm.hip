@@ -182,6 +182,178 @@ __global__ __launch_bounds__(256) void gemm_mfma(const T* __restrict__ A, const 
 }
 
 // ---------------------------------------------------------------------------
+// Real FP64 / FP32 kernel for aligned operands (the H_eff and environment
+// GEMMs): 16-byte global loads, double-buffered LDS (one barrier per k-tile)
+// and an XCD-aware tile order.  Requirements checked on the host: leading
+// dimensions, batch strides and base pointers multiples of 16 bytes.
+// Workgroups are dispatched round-robin over the 8 XCDs; the tile order is
+// remapped so each XCD owns a contiguous run of row-major tiles, i.e. the
+// tiles that share an A row panel share an L2.
+// ---------------------------------------------------------------------------
+template <typename T> struct Vec;
+template <> struct Vec<float> { typedef flt4 type; static constexpr int W = 4; };
+template <> struct Vec<double> { typedef double dbl2 __attribute__((ext_vector_type(2))); typedef dbl2 type; static constexpr int W = 2; };
+
+template <typename T, int TM, int TN, bool TA, bool TB, int TAG>
+__global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const T* __restrict__ B,
+                                                 T* __restrict__ C, GemmArgs g) {
+  constexpr int BM = 32 * TM, BN = 32 * TN, BK = 16;
+  constexpr int PAD = (sizeof(T) == 8) ? 2 : 4;     // keeps rows 16-byte aligned
+  constexpr int VW = Vec<T>::W;
+  typedef typename Vec<T>::type vec_t;
+  constexpr int NA = BM * BK / (VW * 256), NB = BN * BK / (VW * 256);
+  typedef typename Mfma<T>::acc_t acc_t;
+
+  __shared__ __attribute__((aligned(16))) T As[2][BM][BK + PAD];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BN][BK + PAD];
+
+  const long long bz = blockIdx.z;
+  A += bz * g.sA;
+  B += bz * g.sB;
+  C += bz * g.sC;
+  // XCD-aware tile order
+  const int nbx = gridDim.x, nblk = gridDim.x * gridDim.y;
+  int lin = blockIdx.y * nbx + blockIdx.x;
+  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
+  const int m0 = (lin / nbx) * BM, n0 = (lin % nbx) * BN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int M = g.M, N = g.N, K = g.K;
+
+  acc_t acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+
+  vec_t ra[NA], rb[NB];
+
+  auto load_regs = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      const int v = tid + 256 * e;
+      int r, c;  // element (m = m0 + r, k = k0 + c) is the first of the vector
+      if (!TA) { r = v / (BK / VW); c = (v % (BK / VW)) * VW; } else { c = v / (BM / VW); r = (v % (BM / VW)) * VW; }
+      const int gm = m0 + r, gk = k0 + c;
+      if (!TA) {
+        const T* p = A + (long long)gm * g.lda + gk;
+        if (gm < M && gk + VW <= K) {
+          ra[e] = *reinterpret_cast<const vec_t*>(p);
+        } else {
+#pragma unroll
+          for (int j = 0; j < VW; ++j) ra[e][j] = (gm < M && gk + j < K) ? p[j] : T(0);
+        }
+      } else {
+        const T* p = A + (long long)gk * g.lda + gm;
+        if (gk < K && gm + VW <= M) {
+          ra[e] = *reinterpret_cast<const vec_t*>(p);
+        } else {
+#pragma unroll
+          for (int j = 0; j < VW; ++j) ra[e][j] = (gk < K && gm + j < M) ? p[j] : T(0);
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int v = tid + 256 * e;
+      int r, c;  // element (n = n0 + r, k = k0 + c)
+      if (!TB) { c = v / (BN / VW); r = (v % (BN / VW)) * VW; } else { r = v / (BK / VW); c = (v % (BK / VW)) * VW; }
+      const int gn = n0 + r, gk = k0 + c;
+      if (!TB) {
+        const T* p = B + (long long)gk * g.ldb + gn;
+        if (gk < K && gn + VW <= N) {
+          rb[e] = *reinterpret_cast<const vec_t*>(p);
+        } else {
+#pragma unroll
+          for (int j = 0; j < VW; ++j) rb[e][j] = (gk < K && gn + j < N) ? p[j] : T(0);
+        }
+      } else {
+        const T* p = B + (long long)gn * g.ldb + gk;
+        if (gn < N && gk + VW <= K) {
+          rb[e] = *reinterpret_cast<const vec_t*>(p);
+        } else {
+#pragma unroll
+          for (int j = 0; j < VW; ++j) rb[e][j] = (gn < N && gk + j < K) ? p[j] : T(0);
+        }
+      }
+    }
+  };
+  auto store_regs = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      const int v = tid + 256 * e;
+      if (!TA) {
+        const int r = v / (BK / VW), c = (v % (BK / VW)) * VW;
+        *reinterpret_cast<vec_t*>(&As[buf][r][c]) = ra[e];
+      } else {
+        const int c = v / (BM / VW), r = (v % (BM / VW)) * VW;
+#pragma unroll
+        for (int j = 0; j < VW; ++j) As[buf][r + j][c] = ra[e][j];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int v = tid + 256 * e;
+      if (!TB) {
+        const int c = v / (BN / VW), r = (v % (BN / VW)) * VW;
+#pragma unroll
+        for (int j = 0; j < VW; ++j) Bs[buf][r + j][c] = rb[e][j];
+      } else {
+        const int r = v / (BK / VW), c = (v % (BK / VW)) * VW;
+        *reinterpret_cast<vec_t*>(&Bs[buf][r][c]) = rb[e];
+      }
+    }
+  };
+
+  load_regs(0);
+  store_regs(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    const bool more = k0 + BK < K;
+    if (more) load_regs(k0 + BK);
+    T fa[TM][4], fb[TN][4];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) fa[a][t] = As[cur][wm * TM * 16 + a * 16 + li][lk * 4 + t];
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) fb[b][t] = Bs[cur][wn * TN * 16 + b * 16 + li][lk * 4 + t];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = Mfma<T>::op(fa[a][t], fb[b][t], acc[a][b]);
+    if (more) store_regs(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  const T alpha = T(g.alpha), beta = T(g.beta);
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = (sizeof(T) == 8) ? (4 * r + lk) : (lk * 4 + r);
+        const int row = m0 + wm * TM * 16 + a * 16 + rr;
+        const int col = n0 + wn * TN * 16 + b * 16 + li;
+        if (row < M && col < N) {
+          const long long off = (long long)row * g.ldc + col;
+          T v = alpha * acc[a][b][r];
+          if (g.beta != 0.0) v += beta * C[off];
+          C[off] = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
 // split-BF16x3 kernel: FP32 operands, x = hi + lo with hi = bf16(x) and
 // lo = bf16(x - hi); C += Ahi*Bhi + Ahi*Blo + Alo*Bhi with FP32 accumulation.
 // The split happens once per element when the tile is written to LDS.
@@ -396,6 +568,37 @@ static void launch_tt(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
   else launch_tagged<T, CX, TM, TN, true, true>(c, d, A, B, C);
 }
 
+template <typename T, int TM, int TN, bool TA, bool TB>
+static void launch_fast_tagged(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
+  GemmArgs g = make_args(d);
+  dim3 grid((d.N + 32 * TN - 1) / (32 * TN), (d.M + 32 * TM - 1) / (32 * TM), d.batch);
+  DMRG_REQUIRE(grid.y <= 65535, "gemm: M too large for one launch");
+  switch (d.tag) {
+#define CASE(t) \
+  case t: gemm_fast<T, TM, TN, TA, TB, t><<<grid, 256, 0, c->stream>>>(A, B, C, g); break;
+    DMRG_GEMM_TAGS(CASE)
+#undef CASE
+    default: throw Error("gemm: bad tag");
+  }
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
+template <typename T, int TM, int TN>
+static void launch_fast(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
+  if (!d.transA && !d.transB) launch_fast_tagged<T, TM, TN, false, false>(c, d, A, B, C);
+  else if (d.transA && !d.transB) launch_fast_tagged<T, TM, TN, true, false>(c, d, A, B, C);
+  else if (!d.transA && d.transB) launch_fast_tagged<T, TM, TN, false, true>(c, d, A, B, C);
+  else launch_fast_tagged<T, TM, TN, true, true>(c, d, A, B, C);
+}
+
+// 16-byte vector loads are legal when every row start is 16-byte aligned
+template <typename T>
+static bool vec_ok(const GemmDesc& d, const T* A, const T* B) {
+  const int vw = 16 / (int)sizeof(T);
+  auto al = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  return al(A) && al(B) && d.lda % vw == 0 && d.ldb % vw == 0 && d.sA % vw == 0 && d.sB % vw == 0;
+}
+
 // Tile choice: large problems use 128x128 (real) / 64x64 (complex) tiles;
 // problems that would leave the 256 CUs mostly idle drop to 64x64 / 32x32.
 template <typename T>
@@ -403,6 +606,11 @@ static void gemm_any(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
   check_desc(d);
   if (d.M == 0 || d.N == 0) return;
   long long tiles128 = (long long)((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch;
+  if (d.field == REAL && vec_ok(d, A, B)) {
+    if (tiles128 >= 256) launch_fast<T, 4, 4>(c, d, A, B, C);
+    else launch_fast<T, 2, 2>(c, d, A, B, C);
+    return;
+  }
   if (d.field == CPLX) {
     long long tiles64 = (long long)((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch;
     if (tiles64 >= 512) launch_tt<T, true, 2, 2>(c, d, A, B, C);

