@@ -167,25 +167,37 @@ class MPO:
 
 
 def build_mpo(codes, terms, force_complex=False):
-    """Finite-state-automaton MPO of the given terms (see module docstring)."""
+    """Finite-state-automaton MPO of the given terms (see module docstring).
+
+    Bond b sits between sites b-1 and b (0-based); its states are "start"
+    (identity so far) only if some term begins at a site >= b, "done" only
+    if some term ended at a site < b, and one state per distinct operator
+    prefix crossing the bond.  Dropping unused start/done channels gives
+    the minimal bond dimension for sums of local and two-site terms (the
+    rank the reference's SVD compression reaches), which also keeps the
+    zip-up MPO application from truncating intermediate states."""
     n = len(codes)
     dims = [_sites.dim(c) for c in codes]
     table = MatrixTable()
     tl = compile_terms(codes, terms, table)
-    START, DONE = 0, 1
-    # bond b sits between site b-1 and site b (0-based sites); bonds 1..n-1
-    states = [dict() for _ in range(n + 1)]       # prefix key -> index
-    nstate = [1] + [2] * (n - 1) + [1]
 
     def key_of(t, b):
         return tuple((s, t.mats[s]) for s in range(t.first, b))
 
+    # state index maps per bond: "S" start, "D" done, prefix tuples
+    states = [dict() for _ in range(n + 1)]
+    states[0]["S"] = 0
+    states[n]["D"] = 0
+    for b in range(1, n):
+        if any(t.first >= b for t in tl):
+            states[b]["S"] = len(states[b])
+        if any(t.last < b for t in tl):
+            states[b]["D"] = len(states[b])
     for t in tl:
         for b in range(t.first + 1, t.last + 1):
             k = key_of(t, b)
             if k not in states[b]:
-                states[b][k] = nstate[b]
-                nstate[b] += 1
+                states[b][k] = len(states[b])
     cplx = force_complex or any(abs(t.coef.imag) > 0 for t in tl) or \
         any(np.abs(m.imag).max() > 0 for m in table.mats)
     npl = 2 if cplx else 1
@@ -194,37 +206,28 @@ def build_mpo(codes, terms, force_complex=False):
     # where each coefficient sits; the element equals source * coef * matrix
     coupling_sites = []
     for s in range(n):
-        wl = nstate[s] if s > 0 else 1
-        wr = nstate[s + 1] if s < n - 1 else 1
+        left, right = states[s], states[s + 1]
+        wl, wr = max(1, len(left)), max(1, len(right))
         d = dims[s]
         Wc = np.zeros((wl, wr, d, d), dtype=complex)
         eye = np.eye(d)
-        # identity channels (start->start before the term region, done->done after)
-        if s > 0 and s < n - 1:
-            Wc[START, START] += eye
-            Wc[DONE, DONE] += eye
-        elif s == 0 and n > 1:
-            Wc[0, START] += eye
-        elif s == n - 1 and n > 1:
-            Wc[DONE, 0] += eye
+        if "S" in left and "S" in right:
+            Wc[left["S"], right["S"]] += eye
+        if "D" in left and "D" in right:
+            Wc[left["D"], right["D"]] += eye
         for t in tl:
             if s < t.first or s > t.last:
                 continue
             mid = t.mats[s]
             m = eye if mid < 0 else table.mats[mid]
-            if s == t.first:
-                fr = 0 if s == 0 else START
-            else:
-                fr = states[s][key_of(t, s)]
+            fr = left["S"] if s == t.first else left[key_of(t, s)]
             if s == t.last:
-                to = 0 if s == n - 1 else DONE
+                to = right["D"]
                 Wc[fr, to] += t.coef * m
                 coupling_sites.append((t.index, s, fr, to, m, t.source))
             else:
-                to = states[s + 1][key_of(t, s + 1)]
+                to = right[key_of(t, s + 1)]
                 Wc[fr, to] = m
-        if n == 1:
-            Wc = Wc[:1, :1]
         planes = [Wc.real] + ([Wc.imag] if cplx else [])
         W.append(np.ascontiguousarray(np.stack(planes), dtype=np.float64))
     return MPO(W, codes, coupling_sites)

@@ -106,3 +106,25 @@ def test_front_end_jordan_wigner_matches_reference_terms():
         H = H + 0.5 * obj2MO([["N", i], ["N", i + 1]])
     diff = ed.sparse_hamiltonian(codes, H.to_terms()) - ed.sparse_hamiltonian(codes, ref_terms)
     assert abs(diff).max() < 1e-13
+
+
+def test_pruned_automaton_bond_dims_and_exact_zipup():
+    """Unused start/done channels are dropped: Heisenberg gets 4,5,...,5,4,
+    a single-site operator w=1, and the zip-up application with the
+    product MPO (oracle zip-up on the same tensors) is exact at the same
+    maxdim as the reference's compressed MPO."""
+    from oracle import mps as om
+    from dmrgpy_amd.multioperator import heisenberg_terms
+    assert pmpo.build_mpo([2] * 8, heisenberg_terms(8)).bond_dims() == [4, 5, 5, 5, 5, 5, 4]
+    assert pmpo.build_mpo([2] * 4, [(1.0, [("Sz", 2)])]).bond_dims() == [1, 1, 1]
+    n = 6
+    np.random.seed(6)
+    psi = om.random_mps([2] * n, 8)
+    om.position(psi, 1)
+    terms = [(1.0, [("Sz", 3)]), (0.5, [("Sx", 2), ("Sx", 5)])]
+    m = pmpo.build_mpo([2] * n, terms)
+    assert m.bond_dims() == [1, 2, 2, 2, 1]
+    W = [(w[0] if w.shape[0] == 1 else w[0] + 1j * w[1]).transpose(0, 3, 2, 1) for w in m.W]
+    out = om.apply_mpo(W, psi, 1e-12, 8).to_dense()
+    exact = ompo.dense_matrix([2] * n, terms) @ psi.to_dense()
+    assert np.linalg.norm(out - exact) < 1e-13 * np.linalg.norm(exact)
