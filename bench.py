@@ -1,20 +1,26 @@
-"""Benchmark: two-site DMRG sweeps of the open S=1/2 Heisenberg chain (N=100,
-d=2, w=5) at bond dimension D=1024 on one MI355X, reduced-precision
-contractions against the FP64 path from the same state.
+"""Benchmark: two-site DMRG sweeps at bond dimension D=1024 on one MI355X,
+reduced-precision contractions against the FP64 path from the same state.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|bf16x3]
+                    [--model hubbard|heisenberg]
 
-A step is one full DMRG sweep (left-to-right and back, 2(N-1) bond
-updates: Lanczos ground state of H_eff, FP64 SVD truncation, environment
-update).  Setup (untimed): random MPS (seed 0), a D ramp 64 -> 1024 and W
-warm-up sweeps at D=1024.  Timed: K sweeps in the chosen precision; then
-K sweeps in FP64 from the same starting state for the comparison
-(|dE|/N between the two final FP64 Rayleigh quotients).  H_eff throughput
-comes from HIP events around every H_eff apply on the engine's stream.
-The CPU baseline times the oracle (NumPy complex128, the reference
-algorithm) on a bounded sample of the same workload: one H_eff matvec, one
-truncated SVD and one environment update at the D=1024 bond shape,
-combined with the measured Lanczos iteration count per bond.
+Default workload (the configuration that actually reaches D=1024 at the
+reference's default cutoff 1e-12): spinful Hubbard chain on native d=4 sites,
+N=64, t=1, U=4 (particle-hole symmetric form), open boundaries, w=6, D ramped
+128 -> 1024.  ``--model heisenberg`` runs the open S=1/2 Heisenberg chain
+(N=100, d=2, w=5), whose ground state saturates near D=200 at that cutoff.
+
+A step is one full DMRG sweep (left-to-right and back, 2(N-1) bond updates:
+Lanczos ground state of H_eff, FP64 SVD truncation, environment update).
+Setup (untimed): seeded random MPS, the D ramp and W warm-up sweeps at D.
+Timed: K sweeps in the chosen precision; then K sweeps in FP64 from the same
+starting state for the comparison (|dE|/N between the two final FP64
+Rayleigh quotients).  H_eff throughput comes from HIP events recorded
+around every H_eff apply on the engine's stream.  The CPU baseline times the
+oracle (NumPy complex128, the reference algorithm) on a bounded sample of
+the same workload: one H_eff matvec, one truncated SVD and one environment
+update at the D=1024 bond shape, combined with the measured Lanczos
+iteration count per bond.
 """
 
 import argparse
@@ -40,7 +46,7 @@ import json, os, sys, time
 import numpy as np
 sys.path.insert(0, sys.argv[1])
 from oracle import dmrg as od, linalg as ol, mps as om
-D, d, w = int(sys.argv[2]), 2, 5
+D, d, w = int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4])
 rng = np.random.default_rng(0)
 c = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
 L, R = c(D, w, D), c(D, w, D)
@@ -48,7 +54,6 @@ W1, W2 = c(w, d, d, w), c(w, d, d, w)
 th = c(D, d, d, D)
 mv = od.heff2_matvec(L, W1, W2, R, th.shape)
 x = th.reshape(-1)
-mv(x)
 t0 = time.perf_counter(); y = mv(x); t_mv = time.perf_counter() - t0
 t0 = time.perf_counter(); ol.svd_truncated(th.reshape(D * d, d * D), 1e-12, D, 1); t_svd = time.perf_counter() - t0
 A = c(D, d, D)
@@ -57,13 +62,13 @@ print(json.dumps({"t_matvec": t_mv, "t_svd": t_svd, "t_env": t_env}))
 """
 
 
-def cpu_baseline(D, n_sites, iters_per_bond, threads):
+def cpu_baseline(D, d, w, n_sites, iters_per_bond, threads):
     env = dict(os.environ)
     for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS"):
         env[k] = str(threads)
     t0 = time.time()
-    p = subprocess.run([sys.executable, "-c", CPU_SAMPLE, ROOT, str(D)], env=env, capture_output=True,
-                       text=True, timeout=900)
+    p = subprocess.run([sys.executable, "-c", CPU_SAMPLE, ROOT, str(D), str(d), str(w)], env=env,
+                       capture_output=True, text=True, timeout=1200)
     if p.returncode != 0:
         log("cpu baseline failed:", p.stderr[-2000:])
         return None
@@ -72,10 +77,11 @@ def cpu_baseline(D, n_sites, iters_per_bond, threads):
     sweep_s = 2 * (n_sites - 1) * per_bond
     return {"value": 1.0 / sweep_s, "unit": "sweeps/s", "cores": threads, "kind": "port",
             "sample": ("oracle (NumPy complex128 restatement of the reference algorithm) timed on one "
-                       "D={} bond: 1 H_eff matvec {:.2f}s, 1 truncated SVD {:.2f}s, 1 env update {:.2f}s; "
-                       "sweep = 2(N-1) x ({:.1f} matvecs/bond measured on the GPU run + SVD + env) = {:.0f}s "
-                       "(bulk-bond extrapolation), {} BLAS threads, wall {:.0f}s").format(
-                D, r["t_matvec"], r["t_svd"], r["t_env"], iters_per_bond, sweep_s, threads, time.time() - t0),
+                       "D={} d={} w={} bond: 1 H_eff matvec {:.2f}s, 1 truncated SVD {:.2f}s, 1 env update "
+                       "{:.2f}s; sweep = 2(N-1) x ({:.1f} matvecs/bond as measured on the GPU run + SVD + env) "
+                       "= {:.0f}s (bulk-bond extrapolation), {} BLAS threads, wall {:.0f}s").format(
+                D, d, w, r["t_matvec"], r["t_svd"], r["t_env"], iters_per_bond, sweep_s, threads,
+                time.time() - t0),
             "sweep_s": sweep_s}
 
 
@@ -99,15 +105,25 @@ def run_sweeps(ch, psi, K, prec_name, maxdim):
     e = (alg.expectation(eng, psi, ch.H, psi) / alg.overlap(eng, psi, psi)).real
     return {"sweep_s": dt / K, "energy": e, "heff_ms": tms[0], "heff_flops": flops[0], "heff_calls": calls[0],
             "env_ms": tms[1], "svd_ms": tms[2], "svd_calls": calls[2],
-            "iters_per_bond": sum(stats.lanczos_iters) / max(1, len(stats.lanczos_iters))}
+            "iters_per_bond": sum(stats.lanczos_iters) / max(1, len(stats.lanczos_iters)),
+            "max_bond": max(psi.bond_dims())}
+
+
+MODELS = {
+    "hubbard": {"n": 64, "code": 1, "d": 4, "w": 6, "ramp": [128, 256, 512],
+                "desc": "spinful Hubbard chain (native d=4 sites), N={n}, t=1, U=4, open, w=6"},
+    "heisenberg": {"n": 100, "code": 2, "d": 2, "w": 5, "ramp": [64, 128, 256, 512],
+                   "desc": "open S=1/2 Heisenberg chain, N={n}, J=1, d=2, w=5"},
+}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=100)
+    ap.add_argument("--model", default="hubbard", choices=sorted(MODELS))
+    ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--D", type=int, default=1024)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--no-fp64", action="store_true")
@@ -120,18 +136,20 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     from dmrgpy_amd import algorithms as alg
     from dmrgpy_amd.chain import Chain, RES_TOL
     from dmrgpy_amd.engine import PREC_NAMES
-    from dmrgpy_amd.multioperator import heisenberg_terms
+    from dmrgpy_amd.multioperator import heisenberg_terms, hubbard_terms
 
-    n, D = args.n, args.D
-    ch = Chain([2] * n, precision=args.precision, field="real")
+    spec = MODELS[args.model]
+    n = args.n or spec["n"]
+    D = args.D
+    terms = hubbard_terms(n, 1.0, 4.0) if args.model == "hubbard" else heisenberg_terms(n)
+    ch = Chain([spec["code"]] * n, precision=args.precision, field="real")
     ch.set_sweep_params(D, 1, 1e-12, 0.0)
-    ch.set_hamiltonian(heisenberg_terms(n))
+    ch.set_hamiltonian(terms)
     np.random.seed(0)
-    ramp = [m for m in (64, 128, 256, 512) if m < D]
+    ramp = [m for m in spec["ramp"] if m < D]
     psi = ch._default_mps(maxm=ramp[0] if ramp else D)
     eng = ch.eng
     t_setup = time.perf_counter()
@@ -159,8 +177,9 @@ def main():
     peak = PEAK_TFLOPS[args.precision]
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(D, n, res["iters_per_bond"], args.cpu_threads)
+        cpu = cpu_baseline(D, spec["d"], spec["w"], n, res["iters_per_bond"], args.cpu_threads)
         log("cpu baseline: {}".format(cpu))
+    achieved = flops_per / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": 1.0 / res["sweep_s"],
@@ -172,29 +191,34 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-contract/fp64-krylov-svd" if args.precision == "fp32" else "bf16x3-contract/fp64-krylov-svd",
-        "data": "synthetic: seeded random initial MPS, open S=1/2 Heisenberg chain",
-        "config": {"workload": "two-site DMRG sweep, open S=1/2 Heisenberg N={}, d=2, w=5, D={} "
-                               "(ramp 64->{} then warm-up), {} contractions".format(n, D, D, args.precision),
-                   "N": n, "D": D, "d": 2, "w": 5, "precision": args.precision, "field": "real"},
+        "dtype": "{}-contract/fp64-krylov-svd".format(args.precision),
+        "data": "synthetic: seeded random initial MPS (np.random.seed(0)), model Hamiltonian",
+        "config": {"workload": "two-site DMRG sweep at D={}: {}, D ramp {} then {} warm-up sweep(s), "
+                               "{} contractions".format(D, spec["desc"].format(n=n), ramp, args.warmup,
+                                                        args.precision),
+                   "model": args.model, "N": n, "D": D, "d": spec["d"], "w": spec["w"], "cutoff": 1e-12,
+                   "precision": args.precision, "field": "real"},
+        "max_bond": res["max_bond"],
         "heff_tflops": heff_tflops,
         "heff_calls": res["heff_calls"],
         "lanczos_iters_per_bond": res["iters_per_bond"],
         "sweep_s": res["sweep_s"],
-        "time_split_s_per_sweep": {"heff": res["heff_ms"] / 1e3 / args.steps, "env": res["env_ms"] / 1e3 / args.steps,
+        "time_split_s_per_sweep": {"heff": res["heff_ms"] / 1e3 / args.steps,
+                                   "env": res["env_ms"] / 1e3 / args.steps,
                                    "svd": res["svd_ms"] / 1e3 / args.steps},
         "energy": res["energy"],
-        "roofline": {"bound": "mfma", "achieved": flops_per / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0,
-                     "peak": peak, "unit": "TFLOP/s",
-                     "frac": (flops_per / (avg_ms * 1e-3) / 1e12) / peak if avg_ms > 0 else 0.0,
-                     "traffic": None,
-                     "kernel": "H_eff two-site apply (L-GEMM, W small-op, R-GEMM), {:.3g} GFLOP per apply, "
-                               "{:.3f} ms average".format(flops_per / 1e9, avg_ms)},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "H_eff two-site apply (L-GEMM, W small-op, R-GEMM), {:.3g} GFLOP per apply "
+                               "averaged over the sweep's bonds, {:.3f} ms average".format(flops_per / 1e9, avg_ms)},
     }
     if res64 is not None:
         out["fp64"] = {"sweep_s": res64["sweep_s"], "energy": res64["energy"],
                        "heff_tflops": res64["heff_flops"] / (res64["heff_ms"] * 1e-3) / 1e12,
-                       "lanczos_iters_per_bond": res64["iters_per_bond"]}
+                       "lanczos_iters_per_bond": res64["iters_per_bond"],
+                       "time_split_s_per_sweep": {"heff": res64["heff_ms"] / 1e3 / args.steps,
+                                                  "env": res64["env_ms"] / 1e3 / args.steps,
+                                                  "svd": res64["svd_ms"] / 1e3 / args.steps}}
         out["speedup_vs_fp64"] = res64["sweep_s"] / res["sweep_s"]
         out["dE_per_site_vs_fp64"] = abs(res["energy"] - res64["energy"]) / n
     if cpu is not None:

@@ -24,12 +24,13 @@ namespace dmrg {
 // ---------------------------------------------------------------------------
 // small operator: out[x][o][inner] = sum_i Wsm[o][i] * in[x][i][inner], with
 // the (o, i) -> memory offset maps given by ooff / ioff.  One thread per
-// (x, inner) element and a chunk of OCH outputs; W is uniform across the
-// wave and is read through the scalar cache.
+// (x, inner) element and a chunk of OCH outputs (32 real, 16 complex, so the
+// input is re-read KoutPad/OCH times); W is uniform across the wave and is
+// read through the scalar cache.
 // ---------------------------------------------------------------------------
-constexpr int OCH = 16;
+constexpr int OCH_PAD = 32;   // Kout is padded to a multiple of this
 
-template <typename T, bool CXV, bool CXW>
+template <typename T, bool CXV, bool CXW, int OCH>
 __global__ __launch_bounds__(256) void smallop_k(const T* __restrict__ in, T* __restrict__ out,
                                                  const T* __restrict__ Wsm, const int* __restrict__ ioff,
                                                  const int* __restrict__ ooff, int X, int inner,
@@ -115,7 +116,7 @@ void from_prec(Ctx* c, int prec, const void* src, double* dst, long long n) {
 
 size_t prec_size(int prec) { return prec == P64 ? sizeof(double) : sizeof(float); }
 
-// Uploads a small operator (W rows padded to a multiple of OCH) and its
+// Uploads a small operator (W rows padded to a multiple of OCH_PAD) and its
 // offset tables into a workspace slot; returns device pointers.
 SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& wre,
                        const std::vector<double>& wim, bool wcplx, int Kin, int Kout,
@@ -123,7 +124,7 @@ SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& w
   SmallOp s;
   s.Kin = Kin;
   s.Kout = Kout;
-  s.KoutPad = ((Kout + OCH - 1) / OCH) * OCH;
+  s.KoutPad = ((Kout + OCH_PAD - 1) / OCH_PAD) * OCH_PAD;
   s.wcplx = wcplx;
   const size_t es = prec_size(prec);
   const size_t nw = (size_t)s.KoutPad * Kin * (wcplx ? 2 : 1);
@@ -156,20 +157,21 @@ void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, 
                  int inner, long long sx_in, long long sx_out, long long pin, long long pout) {
   if (X <= 0 || inner <= 0 || s.Kout <= 0) return;
   DMRG_REQUIRE(X <= 65535, "smallop: outer extent too large");
-  dim3 grid((inner + 255) / 256, X, s.KoutPad / OCH);
   const bool cv = field == CPLX;
-#define LAUNCH(T, CV, CW)                                                                         \
-  smallop_k<T, CV, CW><<<grid, 256, 0, c->stream>>>((const T*)in, (T*)out, (const T*)s.W, s.ioff, \
-                                                     s.ooff, X, inner, sx_in, sx_out, s.Kin,     \
-                                                     s.Kout, s.KoutPad, pin, pout)
+  const int och = cv ? 16 : 32;
+  dim3 grid((inner + 255) / 256, X, s.KoutPad / och);
+#define LAUNCH(T, CV, CW, O)                                                                         \
+  smallop_k<T, CV, CW, O><<<grid, 256, 0, c->stream>>>((const T*)in, (T*)out, (const T*)s.W, s.ioff, \
+                                                        s.ooff, X, inner, sx_in, sx_out, s.Kin,      \
+                                                        s.Kout, s.KoutPad, pin, pout)
   if (prec == P64) {
-    if (!cv) { DMRG_REQUIRE(!s.wcplx, "smallop: complex MPO needs complex vectors"); LAUNCH(double, false, false); }
-    else if (s.wcplx) LAUNCH(double, true, true);
-    else LAUNCH(double, true, false);
+    if (!cv) { DMRG_REQUIRE(!s.wcplx, "smallop: complex MPO needs complex vectors"); LAUNCH(double, false, false, 32); }
+    else if (s.wcplx) LAUNCH(double, true, true, 16);
+    else LAUNCH(double, true, false, 16);
   } else {
-    if (!cv) { DMRG_REQUIRE(!s.wcplx, "smallop: complex MPO needs complex vectors"); LAUNCH(float, false, false); }
-    else if (s.wcplx) LAUNCH(float, true, true);
-    else LAUNCH(float, true, false);
+    if (!cv) { DMRG_REQUIRE(!s.wcplx, "smallop: complex MPO needs complex vectors"); LAUNCH(float, false, false, 32); }
+    else if (s.wcplx) LAUNCH(float, true, true, 16);
+    else LAUNCH(float, true, false, 16);
   }
 #undef LAUNCH
   DMRG_CHECK_HIP(hipGetLastError());
