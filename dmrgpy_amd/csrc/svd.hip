@@ -9,6 +9,11 @@
 // fallback to a full SVD (rocSOLVER gesvd) applies when the matrix is small
 // (min(m,n) < 16) or the smallest kept singular value is below 1e-7 of the
 // largest (svd.py:108-151).
+// One deliberate difference: the reference divides by S to get the factor on
+// the non-Gram side (svd.py:140-145), whose orthonormality then degrades as
+// eps * (s_0 / s)^2.  Where that factor is the one a sweep leaves behind as
+// an isometry, it is re-orthonormalised here (lowdin below), which keeps
+// H_eff of later bonds a true projection of H.
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -124,6 +129,122 @@ static unsigned g1(long long n) {
   if (b > 4096) b = 4096;
   if (b < 1) b = 1;
   return (unsigned)b;
+}
+
+// T = s * I (k x k planar; the imaginary plane, if any, is zeroed)
+__global__ void set_identity_k(double* __restrict__ T, int k, long long plane, int cx, double s) {
+  long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long tot = (long long)k * k;
+  for (; idx < tot; idx += (long long)gridDim.x * 256) {
+    T[idx] = (idx / k == idx % k) ? s : 0.0;
+    if (cx) T[idx + plane] = 0.0;
+  }
+}
+
+// out[0] = max |T - I| over a k x k planar matrix (single block, fixed order)
+__global__ __launch_bounds__(256) void dev_identity_k(const double* __restrict__ T, int k, long long plane, int cx,
+                                                      double* __restrict__ out) {
+  __shared__ double red[256];
+  double m = 0.0;
+  const long long tot = (long long)k * k;
+  for (long long i = threadIdx.x; i < tot; i += 256) {
+    m = fmax(m, fabs(T[i] - ((i / k == i % k) ? 1.0 : 0.0)));
+    if (cx) m = fmax(m, fabs(T[i + plane]));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Loewdin re-orthonormalisation of the factor a Gram-route split derives by
+// dividing by the singular values.  In exact arithmetic its rows (rows:
+// X is keep x len) or columns (X is len x keep) are orthonormal; in floating
+// point they carry errors of order eps * (s_0 / s)^2, which for singular
+// values near the 1e-7 s_0 guard reaches 1e-2 and breaks the isometry the
+// environments of the next bonds rely on.  With O = X X^dag (X^dag X) = I + E,
+// the coupled Newton-Schulz iteration  T = (3I - Z Y) / 2,  Y <- Y T,
+// Z <- T Z  from Y = O, Z = I converges quadratically to Y = O^{1/2},
+// Z = O^{-1/2}.  X becomes Z X (X Z), the closest matrix with orthonormal
+// rows (columns), and the partner factor P becomes P Y (Y P), so the
+// product of the two factors is unchanged.  Returns false, with nothing
+// modified, when the iteration does not converge.
+static bool lowdin(Ctx* c, int field, double* X, int keep, int len, bool rows, double* P, int plen) {
+  const bool cx = field == CPLX;
+  const size_t es = cx ? 2 : 1;
+  const long long kk = (long long)keep * keep;
+  const long long pX = (long long)keep * len, pP = (long long)keep * plen;
+  double* buf = (double*)ws_get(c, WS_SVD_FIX, sizeof(double) * (es * 5 * (size_t)kk + 16));
+  double* Y = buf;
+  double* Z = Y + es * kk;
+  double* T = Z + es * kk;
+  double* Yn = T + es * kk;
+  double* Zn = Yn + es * kk;
+  double* dev = Zn + es * kk;
+  auto sq = [&](const double* A, const double* B, double* C, double alpha, double beta) {
+    GemmDesc d = gdesc(keep, keep, keep, keep, keep, keep, field, 4);
+    d.pA = d.pB = d.pC = kk;
+    d.alpha = alpha;
+    d.beta = beta;
+    gemm_f64(c, d, A, B, C);
+  };
+  {  // Y = O
+    GemmDesc d = gdesc(keep, keep, len, rows ? len : keep, rows ? len : keep, keep, field, 4);
+    if (rows) { d.transB = true; d.conjB = true; } else { d.transA = true; d.conjA = true; }
+    d.pA = d.pB = pX;
+    d.pC = kk;
+    gemm_f64(c, d, X, X, Y);
+  }
+  set_identity_k<<<g1(kk), 256, 0, c->stream>>>(Z, keep, kk, cx, 1.0);
+  DMRG_CHECK_HIP(hipGetLastError());
+  double prev = 1e300;
+  bool ok = false;
+  for (int it = 0; it < 16; ++it) {
+    set_identity_k<<<g1(kk), 256, 0, c->stream>>>(T, keep, kk, cx, 1.5);
+    DMRG_CHECK_HIP(hipGetLastError());
+    sq(Z, Y, T, -0.5, 1.0);
+    dev_identity_k<<<1, 256, 0, c->stream>>>(T, keep, kk, cx, dev);
+    DMRG_CHECK_HIP(hipGetLastError());
+    double h = 0.0;
+    DMRG_CHECK_HIP(hipMemcpyAsync(&h, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (!(h < 0.25 || (it > 0 && h < prev))) break;   // |E| too large, or diverging
+    if (h < 1e-15 || (h < 1e-12 && h > 0.5 * prev)) {  // converged (or at the rounding floor)
+      ok = true;
+      break;
+    }
+    prev = h;
+    sq(Y, T, Yn, 1.0, 0.0);
+    sq(T, Z, Zn, 1.0, 0.0);
+    std::swap(Y, Yn);
+    std::swap(Z, Zn);
+  }
+  if (!ok) return false;
+  const long long big = std::max((long long)len, (long long)plen) * keep;
+  double* tmp = (double*)ws_get(c, WS_SVD_GRAM, sizeof(double) * es * (size_t)big);
+  if (rows) {  // X <- Z X (keep x len), P <- P Y (plen x keep)
+    GemmDesc d = gdesc(keep, len, keep, keep, len, len, field, 4);
+    d.pA = kk; d.pB = pX; d.pC = pX;
+    gemm_f64(c, d, Z, X, tmp);
+    DMRG_CHECK_HIP(hipMemcpyAsync(X, tmp, sizeof(double) * es * pX, hipMemcpyDeviceToDevice, c->stream));
+    GemmDesc e = gdesc(plen, keep, keep, keep, keep, keep, field, 4);
+    e.pA = pP; e.pB = kk; e.pC = pP;
+    gemm_f64(c, e, P, Y, tmp);
+    DMRG_CHECK_HIP(hipMemcpyAsync(P, tmp, sizeof(double) * es * pP, hipMemcpyDeviceToDevice, c->stream));
+  } else {     // X <- X Z (len x keep), P <- Y P (keep x plen)
+    GemmDesc d = gdesc(len, keep, keep, keep, keep, keep, field, 4);
+    d.pA = pX; d.pB = kk; d.pC = pX;
+    gemm_f64(c, d, X, Z, tmp);
+    DMRG_CHECK_HIP(hipMemcpyAsync(X, tmp, sizeof(double) * es * pX, hipMemcpyDeviceToDevice, c->stream));
+    GemmDesc e = gdesc(keep, plen, keep, keep, plen, plen, field, 4);
+    e.pA = kk; e.pB = pP; e.pC = pP;
+    gemm_f64(c, e, Y, P, tmp);
+    DMRG_CHECK_HIP(hipMemcpyAsync(P, tmp, sizeof(double) * es * pP, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return true;
 }
 
 // The reference's keep rule on a descending spectrum.
@@ -249,15 +370,16 @@ static SvdOut svd_full(Ctx* c, int field, const double* M, int m, int n, double 
   return o;
 }
 
-SvdOut svd_split(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
-                 int absorb, double* U, double* V, int kcap, std::vector<double>& svals) {
-  DMRG_REQUIRE(m > 0 && n > 0, "svd: empty matrix");
-  DMRG_REQUIRE(absorb >= 0 && absorb <= 2, "svd: absorb must be 0, 1 or 2");
+// Gram route with the eigenproblem on the left (M M^dag, side 0) or right
+// (M^dag M, side 1) index; side -1 picks the smaller one, as the reference
+// does.  A forced side larger than min(m, n) keeps the min(m, n) largest
+// eigenvalues (the rest are rounding noise of a rank-deficient Gram).
+static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mind,
+                       int absorb, double* U, double* V, int kcap, std::vector<double>& svals, int side) {
   const bool cx = field == CPLX;
-  const int mind = std::max(1, mindim);
-  if (std::min(m, n) < 16) return svd_full(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals);
-  const bool left = m <= n;
+  const bool left = side < 0 ? m <= n : side == 0;
   const int g = left ? m : n;
+  const int rank = std::min(m, n);
   const long long mn = (long long)m * n;
   const size_t es = cx ? 2 : 1;
   double* ev = (double*)ws_get(c, WS_SVD_EV, sizeof(double) * (2 * (size_t)g + 16));
@@ -305,8 +427,8 @@ SvdOut svd_split(Ctx* c, int field, const double* M, int m, int n, double cutoff
   DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
   DMRG_REQUIRE(hinfo == 0, "syevd/heevd did not converge");
   const double scale = hmax > 0.0 ? hmax : 1.0;
-  svals.resize(g);
-  for (int i = 0; i < g; ++i) svals[i] = scale * std::sqrt(std::max(evh[g - 1 - i], 0.0));
+  svals.resize(rank);
+  for (int i = 0; i < rank; ++i) svals[i] = scale * std::sqrt(std::max(evh[g - 1 - i], 0.0));
   SvdOut o;
   o.keep = truncate_rule(svals, cutoff, maxdim, mind, &o.discarded);
   const int keep = o.keep;
@@ -355,10 +477,28 @@ SvdOut svd_split(Ctx* c, int field, const double* M, int m, int n, double cutoff
       }
     }
     DMRG_CHECK_HIP(hipGetLastError());
+    // The factor that stays behind in a sweep (U when S goes right, Vh when
+    // S goes left) must be an isometry.  When the Gram side made it the
+    // derived factor (divided by S), re-orthonormalise it; if that fails,
+    // redo the split with the Gram on its side, where it is an
+    // eigenvector matrix.
+    bool fixed = true;
+    if (left && absorb == 1) fixed = lowdin(c, field, V, keep, n, true, U, m);
+    else if (!left && absorb == 0) fixed = lowdin(c, field, U, keep, m, false, V, n);
     // sk / E must outlive the async copies and kernels above
     DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (!fixed) return svd_gram(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals, left ? 1 : 0);
   }
   return o;
+}
+
+SvdOut svd_split(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
+                 int absorb, double* U, double* V, int kcap, std::vector<double>& svals) {
+  DMRG_REQUIRE(m > 0 && n > 0, "svd: empty matrix");
+  DMRG_REQUIRE(absorb >= 0 && absorb <= 2, "svd: absorb must be 0, 1 or 2");
+  const int mind = std::max(1, mindim);
+  if (std::min(m, n) < 16) return svd_full(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals);
+  return svd_gram(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals, -1);
 }
 
 int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,

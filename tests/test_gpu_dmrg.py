@@ -199,3 +199,34 @@ def test_apply_mpo_vs_oracle():
     v, r = out.to_dense(), ref.to_dense()
     assert np.linalg.norm(v - r) < 1e-11 * np.linalg.norm(r)
     assert out.center == n
+
+
+def test_reduced_precision_sweeps_keep_isometries():
+    """FP32 contractions leave a noise floor in theta, so cutoff 1e-12 keeps
+    singular values near the Gram-route guard.  The tensors a sweep leaves
+    behind must still be isometries (B B^dag = I), otherwise H_eff of later
+    bonds is no projection of H and FP64 Lanczos finds Ritz values below the
+    ground state.  Pinned by ED: every energy stays above E_0."""
+    from dmrgpy_amd import algorithms as alg
+    from dmrgpy_amd.chain import RES_TOL
+    from dmrgpy_amd.multioperator import hubbard_terms
+    from oracle import ed
+    n = 8
+    terms = hubbard_terms(n, 1.0, 4.0)
+    e_ed = ed.ground_state([1] * n, terms)[0]
+    ch = make_session([1] * n, terms, 256, precision="fp32", field="real")
+    np.random.seed(3)
+    psi = ch._default_mps(maxm=64)
+    for _ in range(3):
+        alg.dmrg_sweep(ch.eng, psi, ch.H, 256, 1e-12, 2, 1, None, res_tol=RES_TOL[1])
+    assert psi.center == 1
+    for i in range(2, n + 1):
+        B = psi.A(i).double().cpu().numpy()
+        B = B[0] + 1j * B[1] if B.shape[0] == 2 else B[0]
+        Bm = B.reshape(B.shape[0], -1)
+        assert np.abs(Bm @ Bm.conj().T - np.eye(Bm.shape[0])).max() < 1e-12, i
+    stats = alg.Stats()
+    e64 = alg.dmrg_sweep(ch.eng, psi, ch.H, 256, 1e-12, 2, 0, stats)
+    e_ray = (alg.expectation(ch.eng, psi, ch.H, psi) / alg.overlap(ch.eng, psi, psi)).real
+    assert e64 > e_ed - 1e-9 and e_ray > e_ed - 1e-9
+    assert abs(e_ray - e_ed) / n < 1e-8

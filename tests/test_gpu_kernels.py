@@ -76,7 +76,7 @@ def W_eng(W):   # oracle W[wl, s_in, s_out, wr] -> engine [wl, wr, s_out, s_in]
 @pytest.mark.parametrize("prec", [0, 1, 2])
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
-@pytest.mark.parametrize("shape", [(37, 53, 29), (160, 96, 200)])
+@pytest.mark.parametrize("shape", [(37, 53, 29), (160, 96, 200), (2048, 2176, 48)])
 def test_gemm(engine, prec, cplx, ta, tb, shape):
     import torch
     M, N, K = shape
@@ -220,10 +220,42 @@ def test_svd(engine, cplx, shape, absorb):
     ref = (Ur * Sr[:keep]) @ Vr
     assert relerr(rec, ref) < 1e-10 + 100 * eps * (smax / skeep) ** 2
     orth = 1e-12 + 100 * eps * (smax / skeep) ** 2
-    if absorb in (0, 2):
+    # the factor a sweep leaves behind (U when S goes right, Vh when S goes
+    # left) is an isometry to rounding whatever the spectrum
+    if absorb == 0:
+        assert np.abs(Uh.conj().T @ Uh - np.eye(keep)).max() < 1e-12
+    if absorb == 1:
+        assert np.abs(Vh @ Vh.conj().T - np.eye(keep)).max() < 1e-12
+    if absorb == 2:
         assert np.abs(Uh.conj().T @ Uh - np.eye(keep)).max() < orth
-    if absorb in (1, 2):
         assert np.abs(Vh @ Vh.conj().T - np.eye(keep)).max() < orth
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("shape", [(120, 200), (200, 120), (160, 160)])
+@pytest.mark.parametrize("absorb", [0, 1])
+def test_svd_isometry_near_guard(engine, cplx, shape, absorb):
+    """Spectrum down to just above the Gram-route guard (1e-7 s_0): the
+    reference's division by S would leave ~1e-2 orthonormality errors in the
+    derived factor; the stay-behind factor must still be an isometry to
+    1e-12 and the product must reconstruct M."""
+    import torch
+    m, n = shape
+    rng = np.random.default_rng(m + 3 * n + absorb)
+    k = min(m, n)
+    U0 = np.linalg.qr(rnd(rng, (m, k), cplx))[0]
+    V0 = np.linalg.qr(rnd(rng, (n, k), cplx))[0]
+    s0 = np.logspace(0, -6.9, k)
+    M = (U0 * s0) @ V0.conj().T
+    U, V, keep, disc, sv = engine.svd_split(int(cplx), dev(engine, M), m, n, 0.0, 0, 1, absorb)
+    torch.cuda.synchronize()
+    assert keep == k
+    Uh, Vh = host(U), host(V)
+    if absorb == 0:
+        assert np.abs(Uh.conj().T @ Uh - np.eye(keep)).max() < 1e-12
+    else:
+        assert np.abs(Vh @ Vh.conj().T - np.eye(keep)).max() < 1e-12
+    assert relerr(Uh @ Vh, M) < 1e-12
 
 
 def _hermitian_heff(rng, cplx, Dl=9, Dr=11, d=2, w=3):
