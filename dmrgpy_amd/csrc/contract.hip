@@ -116,8 +116,9 @@ void from_prec(Ctx* c, int prec, const void* src, double* dst, long long n) {
 
 size_t prec_size(int prec) { return prec == P64 ? sizeof(double) : sizeof(float); }
 
-// Uploads a small operator (W rows padded to a multiple of OCH_PAD) and its
-// offset tables into a workspace slot; returns device pointers.
+// Uploads a small operator (W rows padded to a multiple of OCH_PAD), its
+// offset tables and, for a real W, its compressed rows into a workspace slot;
+// returns device pointers.
 SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& wre,
                        const std::vector<double>& wim, bool wcplx, int Kin, int Kout,
                        const std::vector<int>& ioff, const std::vector<int>& ooff) {
@@ -127,9 +128,39 @@ SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& w
   s.KoutPad = ((Kout + OCH_PAD - 1) / OCH_PAD) * OCH_PAD;
   s.wcplx = wcplx;
   const size_t es = prec_size(prec);
+  auto up = [](size_t b) { return ((b + 255) / 256) * 256; };
+  // compressed rows (real W only)
+  std::vector<int> rowptr(Kout + 1, 0), col;
+  std::vector<double> val;
+  if (!wcplx) {
+    for (int o = 0; o < Kout; ++o) {
+      for (int i = 0; i < Kin; ++i) {
+        const double v = wre[(size_t)o * Kin + i];
+        if (v != 0.0) {
+          col.push_back(i);
+          val.push_back(v);
+        }
+      }
+      rowptr[o + 1] = (int)col.size();
+    }
+  }
+  s.nnz = (int)col.size();
+  // row ranges of the four waves: contiguous, about nnz / 4 (plus one per
+  // row for its store) each
+  int grp[5] = {0, Kout, Kout, Kout, Kout};
+  {
+    const double total = (double)s.nnz + Kout;
+    int g = 1;
+    double run = 0;
+    for (int o = 0; o < Kout && g < 4; ++o) {
+      run += (rowptr[o + 1] - rowptr[o]) + 1;
+      if (run >= total * g / 4.0) grp[g++] = o + 1;
+    }
+  }
   const size_t nw = (size_t)s.KoutPad * Kin * (wcplx ? 2 : 1);
-  const size_t bytes_w = ((nw * es + 255) / 256) * 256;
-  const size_t bytes = bytes_w + sizeof(int) * (Kin + Kout) + 256;
+  const size_t bytes_w = up(nw * es);
+  const size_t bytes_o = up(sizeof(int) * (Kin + Kout + Kout + 1 + 5 + col.size()));
+  const size_t bytes = bytes_w + bytes_o + up(val.size() * es) + 256;
   std::vector<unsigned char> host(bytes, 0);
   for (int pl = 0; pl < (wcplx ? 2 : 1); ++pl) {
     const std::vector<double>& src = pl ? wim : wre;
@@ -141,16 +172,195 @@ SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& w
         else reinterpret_cast<float*>(host.data())[idx] = (float)v;
       }
   }
-  std::memcpy(host.data() + bytes_w, ioff.data(), sizeof(int) * Kin);
-  std::memcpy(host.data() + bytes_w + sizeof(int) * Kin, ooff.data(), sizeof(int) * Kout);
+  int* ints = reinterpret_cast<int*>(host.data() + bytes_w);
+  std::memcpy(ints, ioff.data(), sizeof(int) * Kin);
+  std::memcpy(ints + Kin, ooff.data(), sizeof(int) * Kout);
+  std::memcpy(ints + Kin + Kout, rowptr.data(), sizeof(int) * (Kout + 1));
+  std::memcpy(ints + Kin + 2 * Kout + 1, grp, sizeof(int) * 5);
+  if (!col.empty()) std::memcpy(ints + Kin + 2 * Kout + 6, col.data(), sizeof(int) * col.size());
+  unsigned char* vb = host.data() + bytes_w + bytes_o;
+  for (size_t k = 0; k < val.size(); ++k) {
+    if (prec == P64) reinterpret_cast<double*>(vb)[k] = val[k];
+    else reinterpret_cast<float*>(vb)[k] = (float)val[k];
+  }
   unsigned char* dev = (unsigned char*)ws_get(c, slot, bytes);
   DMRG_CHECK_HIP(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
   // the host vector dies at return: make the copy complete first
   DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  const int* di = reinterpret_cast<const int*>(dev + bytes_w);
   s.W = dev;
-  s.ioff = reinterpret_cast<const int*>(dev + bytes_w);
-  s.ooff = reinterpret_cast<const int*>(dev + bytes_w + sizeof(int) * Kin);
+  s.ioff = di;
+  s.ooff = di + Kin;
+  s.rowptr = di + Kin + Kout;
+  s.grp = di + Kin + 2 * Kout + 1;
+  s.col = di + Kin + 2 * Kout + 6;
+  s.val = dev + bytes_w + bytes_o;
   return s;
+}
+
+// ---------------------------------------------------------------------------
+// Sparse small operator for a real W: out[x][o][b] = sum over the nonzeros
+// (i, w) of row o of  w * in[x][i][b].  One block per (x, 512 inner
+// elements, plane); eight chunks of 64 columns are staged in LDS in turn
+// (prefetched into registers one chunk ahead), each lane owns one column and
+// each wave a contiguous range of output rows with about a quarter of the
+// nonzeros.  Row bounds, column indices and values are uniform across a wave
+// (scalar loads); every output row is written as one 256-byte segment per
+// wave.  HBM traffic: the input once and the output once.
+// ---------------------------------------------------------------------------
+constexpr int SP_BN = 64, SP_LDI = SP_BN + 1, SP_CHUNKS = 8, SP_KMAX = 128, SP_PRE = SP_KMAX * SP_BN / 256;
+
+template <typename T>
+__global__ __launch_bounds__(256) void smallop_sparse_k(const T* __restrict__ in, T* __restrict__ out,
+                                                        const int* __restrict__ rowptr,
+                                                        const int* __restrict__ colidx, const T* __restrict__ val,
+                                                        const int* __restrict__ grp, const int* __restrict__ ioff,
+                                                        const int* __restrict__ ooff, int inner, long long sx_in,
+                                                        long long sx_out, int Kin, long long pin, long long pout) {
+  extern __shared__ double smem_d[];
+  T* Is = reinterpret_cast<T*>(smem_d);                 // [Kin][SP_LDI]
+  int* io = reinterpret_cast<int*>(Is + Kin * SP_LDI);  // [Kin]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
+  const int x = blockIdx.y, pl = blockIdx.z;
+  in += (long long)x * sx_in + pl * pin;
+  out += (long long)x * sx_out + pl * pout;
+  for (int i = tid; i < Kin; i += 256) io[i] = ioff[i];
+  const int r0 = grp[wave], r1 = grp[wave + 1];
+  T pre[SP_PRE];
+  auto fetch = [&](int b0) {
+#pragma unroll
+    for (int u = 0; u < SP_PRE; ++u) {
+      const int e = u * 256 + tid;
+      const int i = e / SP_BN, b = b0 + e % SP_BN;
+      pre[u] = (i < Kin && b < inner) ? in[(long long)io[i] + b] : T(0);
+    }
+  };
+  const int bfirst = blockIdx.x * SP_CHUNKS * SP_BN;
+  __syncthreads();   // offsets ready
+  fetch(bfirst);
+  for (int ch = 0; ch < SP_CHUNKS; ++ch) {
+    const int b0 = bfirst + ch * SP_BN;
+    if (b0 >= inner) break;
+    __syncthreads();   // previous chunk's LDS reads done
+#pragma unroll
+    for (int u = 0; u < SP_PRE; ++u) {
+      const int e = u * 256 + tid;
+      if (e < Kin * SP_BN) Is[(e / SP_BN) * SP_LDI + e % SP_BN] = pre[u];
+    }
+    __syncthreads();
+    if (ch + 1 < SP_CHUNKS && b0 + SP_BN < inner) fetch(b0 + SP_BN);
+    const int b = b0 + lane;
+    for (int o = r0; o < r1; ++o) {
+      T acc = T(0);
+      const int k1 = rowptr[o + 1];
+      for (int k = rowptr[o]; k < k1; ++k) acc += val[k] * Is[colidx[k] * SP_LDI + lane];
+      if (b < inner) out[(long long)ooff[o] + b] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA small operator for FP32 vectors and a real W (every realified MPO):
+// per outer index x the operation is the GEMM  out[x] (Kout x inner) =
+// W (Kout x Kin) * in[x] (Kin x inner)  with row offsets ioff / ooff.  One
+// block per (x, 512 inner elements, plane): W is staged in LDS once, then
+// eight chunks of 64 inner columns are staged in turn; each wave owns 16
+// columns and all ceil(Kout/16) = MT row tiles (v_mfma_f32_16x16x4_f32).
+// Reads the input once and writes the output once.
+// ---------------------------------------------------------------------------
+typedef float flt4 __attribute__((ext_vector_type(4)));
+constexpr int SOP_BN = 64, SOP_LDI = SOP_BN + 4, SOP_CHUNKS = 8, SOP_U = 8;
+constexpr int SOP_KMAX = 128, SOP_PRE = SOP_KMAX * SOP_BN / 256;   // Kin <= SOP_KMAX
+
+static size_t smallop_mfma_lds(int Kin, int Kout) {
+  const int KP = (Kin + 3) / 4 * 4, MT = (Kout + 15) / 16;
+  return sizeof(float) * ((size_t)MT * 16 * (KP + 1) + (size_t)KP * SOP_LDI) + sizeof(int) * (Kin + Kout);
+}
+
+template <int MT>
+__global__ __launch_bounds__(256) void smallop_mfma_k(const float* __restrict__ in, float* __restrict__ out,
+                                                      const float* __restrict__ Wsm, const int* __restrict__ ioff,
+                                                      const int* __restrict__ ooff, int inner, long long sx_in,
+                                                      long long sx_out, int Kin, int Kout, long long pin,
+                                                      long long pout) {
+  extern __shared__ float smem[];
+  const int KP = (Kin + 3) / 4 * 4, LDW = KP + 1;
+  float* Ws = smem;                      // [MT * 16][LDW]
+  float* Is = Ws + MT * 16 * LDW;        // [KP][SOP_LDI]
+  int* io = reinterpret_cast<int*>(Is + KP * SOP_LDI);
+  int* oo = io + Kin;
+  const int tid = threadIdx.x;
+  const int x = blockIdx.y, pl = blockIdx.z;
+  in += (long long)x * sx_in + pl * pin;
+  out += (long long)x * sx_out + pl * pout;
+  for (int i = tid; i < Kin; i += 256) io[i] = ioff[i];
+  for (int o = tid; o < Kout; o += 256) oo[o] = ooff[o];
+  // staging loops issue SOP_U independent loads before the first LDS write,
+  // so a block waits for memory once per SOP_U elements per thread
+  for (int base = 0; base < MT * 16 * KP; base += 256 * SOP_U) {
+    float v[SOP_U];
+#pragma unroll
+    for (int u = 0; u < SOP_U; ++u) {
+      const int e = base + u * 256 + tid;
+      const int o = e / KP, i = e % KP;
+      v[u] = (e < MT * 16 * KP && o < Kout && i < Kin) ? Wsm[(long long)o * Kin + i] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < SOP_U; ++u) {
+      const int e = base + u * 256 + tid;
+      if (e < MT * 16 * KP) Ws[(e / KP) * LDW + e % KP] = v[u];
+    }
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int col0 = wave * 16;
+  // input chunks are prefetched into registers one chunk ahead, so their
+  // loads are in flight while the previous chunk is multiplied
+  float pre[SOP_PRE];
+  auto fetch = [&](int b0) {
+#pragma unroll
+    for (int u = 0; u < SOP_PRE; ++u) {
+      const int e = u * 256 + tid;
+      const int i = e / SOP_BN, b = b0 + e % SOP_BN;
+      pre[u] = (i < Kin && b < inner) ? in[(long long)io[i] + b] : 0.f;
+    }
+  };
+  const int bfirst = blockIdx.x * SOP_CHUNKS * SOP_BN;
+  __syncthreads();   // offsets ready
+  fetch(bfirst);
+  for (int ch = 0; ch < SOP_CHUNKS; ++ch) {
+    const int b0 = bfirst + ch * SOP_BN;
+    if (b0 >= inner) break;
+    __syncthreads();   // W ready; previous chunk's LDS reads done
+#pragma unroll
+    for (int u = 0; u < SOP_PRE; ++u) {
+      const int e = u * 256 + tid;
+      if (e < KP * SOP_BN) Is[(e / SOP_BN) * SOP_LDI + e % SOP_BN] = pre[u];
+    }
+    __syncthreads();
+    if (ch + 1 < SOP_CHUNKS && b0 + SOP_BN < inner) fetch(b0 + SOP_BN);
+    flt4 acc[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) acc[t] = flt4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < KP; k0 += 4) {
+      const float bv = Is[(k0 + lk) * SOP_LDI + col0 + li];
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ws[(t * 16 + li) * LDW + k0 + lk], bv, acc[t], 0, 0, 0);
+    }
+    // accumulator element r of tile t: row t*16 + 4*(lane/16) + r, column lane%16
+    const int b = b0 + col0 + li;
+    if (b < inner) {
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int o = t * 16 + 4 * lk + r;
+          if (o < Kout) out[(long long)oo[o] + b] = acc[t][r];
+        }
+    }
+  }
 }
 
 void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, void* out, int X,
@@ -158,6 +368,38 @@ void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, 
   if (X <= 0 || inner <= 0 || s.Kout <= 0) return;
   DMRG_REQUIRE(X <= 65535, "smallop: outer extent too large");
   const bool cv = field == CPLX;
+  if (!s.wcplx && s.Kin <= SP_KMAX && 8LL * s.nnz <= (long long)s.Kin * s.Kout) {
+    dim3 g((inner + SP_BN * SP_CHUNKS - 1) / (SP_BN * SP_CHUNKS), X, cv ? 2 : 1);
+    const size_t es = prec_size(prec);
+    const size_t lds = es * (size_t)s.Kin * SP_LDI + sizeof(int) * s.Kin;
+    if (prec == P64)
+      smallop_sparse_k<double><<<g, 256, lds, c->stream>>>((const double*)in, (double*)out, s.rowptr, s.col,
+                                                           (const double*)s.val, s.grp, s.ioff, s.ooff, inner, sx_in,
+                                                           sx_out, s.Kin, pin, pout);
+    else
+      smallop_sparse_k<float><<<g, 256, lds, c->stream>>>((const float*)in, (float*)out, s.rowptr, s.col,
+                                                          (const float*)s.val, s.grp, s.ioff, s.ooff, inner, sx_in,
+                                                          sx_out, s.Kin, pin, pout);
+    DMRG_CHECK_HIP(hipGetLastError());
+    return;
+  }
+  const size_t lds = smallop_mfma_lds(s.Kin, s.Kout);
+  if (prec != P64 && !s.wcplx && s.Kout <= 128 && s.Kin <= SOP_KMAX && lds <= 65536) {
+    dim3 g((inner + SOP_BN * SOP_CHUNKS - 1) / (SOP_BN * SOP_CHUNKS), X, cv ? 2 : 1);
+    const int MT = (s.Kout + 15) / 16;
+    switch (MT) {
+#define CASE(m)                                                                                               \
+  case m:                                                                                                     \
+    smallop_mfma_k<m><<<g, 256, lds, c->stream>>>((const float*)in, (float*)out, (const float*)s.W, s.ioff, \
+                                                   s.ooff, inner, sx_in, sx_out, s.Kin, s.Kout, pin, pout);    \
+    break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+      default: throw Error("smallop: bad tile count");
+    }
+    DMRG_CHECK_HIP(hipGetLastError());
+    return;
+  }
   const int och = cv ? 16 : 32;
   dim3 grid((inner + 255) / 256, X, s.KoutPad / och);
 #define LAUNCH(T, CV, CW, O)                                                                         \

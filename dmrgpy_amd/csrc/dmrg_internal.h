@@ -14,6 +14,14 @@ struct SmallOp {
   const int* ooff = nullptr;   // [Kout]
   int Kin = 0, Kout = 0, KoutPad = 0;
   bool wcplx = false;
+  // compressed rows of a real W (MPO tensors are sparse): rowptr[Kout + 1],
+  // col[nnz], val[nnz] (contraction precision), and the row ranges handled
+  // by each of the four waves of a block, balanced by nonzeros: grp[5]
+  const int* rowptr = nullptr;
+  const int* col = nullptr;
+  const void* val = nullptr;
+  const int* grp = nullptr;
+  int nnz = 0;
 };
 
 struct HeffOp {

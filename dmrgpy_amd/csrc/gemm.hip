@@ -38,6 +38,7 @@ struct GemmArgs {
   long long sA, sB, sC, pA, pB, pC;
   double alpha, beta;
   int conjA, conjB;
+  int vec;   // 16-byte vector loads allowed (aligned pointers and strides)
 };
 
 // ---------------------------------------------------------------------------
@@ -182,10 +183,10 @@ __global__ __launch_bounds__(256) void gemm_mfma(const T* __restrict__ A, const 
 }
 
 // ---------------------------------------------------------------------------
-// Real FP64 / FP32 kernel for aligned operands (the H_eff and environment
-// GEMMs): 16-byte global loads, double-buffered LDS (one barrier per k-tile)
-// and an XCD-aware tile order.  Requirements checked on the host: leading
-// dimensions, batch strides and base pointers multiples of 16 bytes.
+// Real FP64 / FP32 kernel (the H_eff and environment GEMMs): 16-byte global
+// loads when the host found base pointers, leading dimensions and batch
+// strides multiples of 16 bytes (g.vec), element loads otherwise;
+// double-buffered LDS (one barrier per k-tile) and an XCD-aware tile order.
 // Workgroups are dispatched round-robin over the 8 XCDs; the tile order is
 // remapped so each XCD owns a contiguous run of row-major tiles, i.e. the
 // tiles that share an A row panel share an L2.
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
       const int gm = m0 + r, gk = k0 + c;
       if (!TA) {
         const T* p = A + (long long)gm * g.lda + gk;
-        if (gm < M && gk + VW <= K) {
+        if (g.vec && gm < M && gk + VW <= K) {
           ra[e] = *reinterpret_cast<const vec_t*>(p);
         } else {
 #pragma unroll
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
         }
       } else {
         const T* p = A + (long long)gk * g.lda + gm;
-        if (gk < K && gm + VW <= M) {
+        if (g.vec && gk < K && gm + VW <= M) {
           ra[e] = *reinterpret_cast<const vec_t*>(p);
         } else {
 #pragma unroll
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
       const int gn = n0 + r, gk = k0 + c;
       if (!TB) {
         const T* p = B + (long long)gk * g.ldb + gn;
-        if (gk < K && gn + VW <= N) {
+        if (g.vec && gk < K && gn + VW <= N) {
           rb[e] = *reinterpret_cast<const vec_t*>(p);
         } else {
 #pragma unroll
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
         }
       } else {
         const T* p = B + (long long)gn * g.ldb + gk;
-        if (gn < N && gk + VW <= K) {
+        if (g.vec && gn < N && gk + VW <= K) {
           rb[e] = *reinterpret_cast<const vec_t*>(p);
         } else {
 #pragma unroll
@@ -531,6 +532,7 @@ static GemmArgs make_args(const GemmDesc& d) {
   g.pA = d.pA; g.pB = d.pB; g.pC = d.pC;
   g.alpha = d.alpha; g.beta = d.beta;
   g.conjA = d.conjA; g.conjB = d.conjB;
+  g.vec = 0;
   return g;
 }
 
@@ -569,8 +571,9 @@ static void launch_tt(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
 }
 
 template <typename T, int TM, int TN, bool TA, bool TB>
-static void launch_fast_tagged(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
+static void launch_fast_tagged(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C, bool vec) {
   GemmArgs g = make_args(d);
+  g.vec = vec ? 1 : 0;
   dim3 grid((d.N + 32 * TN - 1) / (32 * TN), (d.M + 32 * TM - 1) / (32 * TM), d.batch);
   DMRG_REQUIRE(grid.y <= 65535, "gemm: M too large for one launch");
   switch (d.tag) {
@@ -584,11 +587,11 @@ static void launch_fast_tagged(Ctx* c, const GemmDesc& d, const T* A, const T* B
 }
 
 template <typename T, int TM, int TN>
-static void launch_fast(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
-  if (!d.transA && !d.transB) launch_fast_tagged<T, TM, TN, false, false>(c, d, A, B, C);
-  else if (d.transA && !d.transB) launch_fast_tagged<T, TM, TN, true, false>(c, d, A, B, C);
-  else if (!d.transA && d.transB) launch_fast_tagged<T, TM, TN, false, true>(c, d, A, B, C);
-  else launch_fast_tagged<T, TM, TN, true, true>(c, d, A, B, C);
+static void launch_fast(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C, bool vec) {
+  if (!d.transA && !d.transB) launch_fast_tagged<T, TM, TN, false, false>(c, d, A, B, C, vec);
+  else if (d.transA && !d.transB) launch_fast_tagged<T, TM, TN, true, false>(c, d, A, B, C, vec);
+  else if (!d.transA && d.transB) launch_fast_tagged<T, TM, TN, false, true>(c, d, A, B, C, vec);
+  else launch_fast_tagged<T, TM, TN, true, true>(c, d, A, B, C, vec);
 }
 
 // 16-byte vector loads are legal when every row start is 16-byte aligned
@@ -601,24 +604,21 @@ static bool vec_ok(const GemmDesc& d, const T* A, const T* B) {
 
 // Tile choice: large problems use 128x128 (real) / 64x64 (complex) tiles;
 // problems that would leave the 256 CUs mostly idle drop to 64x64 / 32x32.
+// Real operands always take the fast kernel; complex ones the planar one.
 template <typename T>
 static void gemm_any(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
   check_desc(d);
   if (d.M == 0 || d.N == 0) return;
   long long tiles128 = (long long)((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch;
-  if (d.field == REAL && vec_ok(d, A, B)) {
-    if (tiles128 >= 256) launch_fast<T, 4, 4>(c, d, A, B, C);
-    else launch_fast<T, 2, 2>(c, d, A, B, C);
+  if (d.field == REAL) {
+    const bool vec = vec_ok(d, A, B);
+    if (tiles128 >= 256) launch_fast<T, 4, 4>(c, d, A, B, C, vec);
+    else launch_fast<T, 2, 2>(c, d, A, B, C, vec);
     return;
   }
-  if (d.field == CPLX) {
-    long long tiles64 = (long long)((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch;
-    if (tiles64 >= 512) launch_tt<T, true, 2, 2>(c, d, A, B, C);
-    else launch_tt<T, true, 1, 1>(c, d, A, B, C);
-  } else {
-    if (tiles128 >= 256) launch_tt<T, false, 4, 4>(c, d, A, B, C);
-    else launch_tt<T, false, 2, 2>(c, d, A, B, C);
-  }
+  long long tiles64 = (long long)((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch;
+  if (tiles64 >= 512) launch_tt<T, true, 2, 2>(c, d, A, B, C);
+  else launch_tt<T, true, 1, 1>(c, d, A, B, C);
 }
 
 void gemm_f64(Ctx* c, const GemmDesc& d, const double* A, const double* B, double* C) {

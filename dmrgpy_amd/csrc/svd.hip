@@ -103,11 +103,35 @@ __global__ void copy_block_k(const double* __restrict__ src, int lds, int rows, 
   }
 }
 
-// out[0] = max |x_i| over n elements (single block, fixed order)
-__global__ __launch_bounds__(256) void absmax_k(const double* __restrict__ x, long long n, double* __restrict__ out) {
+// Two-pass max reductions (the maximum is order independent, so the result
+// is deterministic): block maxima into part[], then one block over part[].
+// mode 0: max |x_i| over n elements; mode 1: max |T - I| over a k x k planar
+// matrix (n = k * k, imaginary plane at +plane when cx).
+__global__ __launch_bounds__(256) void max_partial_k(const double* __restrict__ x, long long n, int mode, int k,
+                                                     long long plane, int cx, double* __restrict__ part) {
   __shared__ double red[256];
   double m = 0.0;
-  for (long long i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(x[i]));
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    if (mode == 0) {
+      m = fmax(m, fabs(x[i]));
+    } else {
+      m = fmax(m, fabs(x[i] - ((i / k == i % k) ? 1.0 : 0.0)));
+      if (cx) m = fmax(m, fabs(x[i + plane]));
+    }
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void max_final_k(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) m = fmax(m, part[i]);
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -115,6 +139,17 @@ __global__ __launch_bounds__(256) void absmax_k(const double* __restrict__ x, lo
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = red[0];
+}
+
+static void dev_max(Ctx* c, const double* x, long long n, int mode, int k, long long plane, int cx, double* out) {
+  long long nb = (n + 4095) / 4096;
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  double* part = (double*)ws_get(c, WS_SVD_RED, sizeof(double) * 1024);
+  max_partial_k<<<(unsigned)nb, 256, 0, c->stream>>>(x, n, mode, k, plane, cx, part);
+  DMRG_CHECK_HIP(hipGetLastError());
+  max_final_k<<<1, 256, 0, c->stream>>>(part, (int)nb, out);
+  DMRG_CHECK_HIP(hipGetLastError());
 }
 
 __global__ void scaled_copy_k(const double* __restrict__ x, long long n, const double* __restrict__ amax,
@@ -139,25 +174,6 @@ __global__ void set_identity_k(double* __restrict__ T, int k, long long plane, i
     T[idx] = (idx / k == idx % k) ? s : 0.0;
     if (cx) T[idx + plane] = 0.0;
   }
-}
-
-// out[0] = max |T - I| over a k x k planar matrix (single block, fixed order)
-__global__ __launch_bounds__(256) void dev_identity_k(const double* __restrict__ T, int k, long long plane, int cx,
-                                                      double* __restrict__ out) {
-  __shared__ double red[256];
-  double m = 0.0;
-  const long long tot = (long long)k * k;
-  for (long long i = threadIdx.x; i < tot; i += 256) {
-    m = fmax(m, fabs(T[i] - ((i / k == i % k) ? 1.0 : 0.0)));
-    if (cx) m = fmax(m, fabs(T[i + plane]));
-  }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // Loewdin re-orthonormalisation of the factor a Gram-route split derives by
@@ -206,8 +222,7 @@ static bool lowdin(Ctx* c, int field, double* X, int keep, int len, bool rows, d
     set_identity_k<<<g1(kk), 256, 0, c->stream>>>(T, keep, kk, cx, 1.5);
     DMRG_CHECK_HIP(hipGetLastError());
     sq(Z, Y, T, -0.5, 1.0);
-    dev_identity_k<<<1, 256, 0, c->stream>>>(T, keep, kk, cx, dev);
-    DMRG_CHECK_HIP(hipGetLastError());
+    dev_max(c, T, kk, 1, keep, kk, cx, dev);
     double h = 0.0;
     DMRG_CHECK_HIP(hipMemcpyAsync(&h, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
@@ -315,8 +330,7 @@ static SvdOut svd_full(Ctx* c, int field, const double* M, int m, int n, double 
   // scale by max|M| so the factorisation never sees overflowing norms
   // (un-normalised MPS tensors reach 1e100 during canonicalisation)
   double* amax = (double*)ws_get(c, WS_SVD_EV, sizeof(double) * (2 * (size_t)k + 16));
-  absmax_k<<<1, 256, 0, c->stream>>>(M, (long long)es * mn, amax);
-  DMRG_CHECK_HIP(hipGetLastError());
+  dev_max(c, M, (long long)es * mn, 0, 0, 0, 0, amax);
   if (cx) {
     double* Ms = (double*)ws_get(c, WS_SVD_TMP2, sizeof(double) * es * mn);
     scaled_copy_k<<<g1(es * mn), 256, 0, c->stream>>>(M, es * mn, amax, Ms);
@@ -393,7 +407,7 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
     // Gram of M / max|M|: the eigensolver never sees overflowing norms
     // (un-normalised MPS tensors reach 1e100 during canonicalisation)
     double* Ms = (double*)ws_get(c, WS_SVD_TMP, sizeof(double) * es * mn);
-    absmax_k<<<1, 256, 0, c->stream>>>(M, (long long)es * mn, amax_dev);
+    dev_max(c, M, (long long)es * mn, 0, 0, 0, 0, amax_dev);
     scaled_copy_k<<<g1(es * mn), 256, 0, c->stream>>>(M, es * mn, amax_dev, Ms);
     DMRG_CHECK_HIP(hipGetLastError());
     GemmDesc d = gdesc(g, g, left ? n : m, 0, 0, g, field, 4);

@@ -140,6 +140,67 @@ def test_heff2(engine, prec, cplx):
     assert err < TOL[prec] * 10, err
 
 
+@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("dims", [(70, 66, 4, 4, 6, 6, 6), (40, 130, 3, 3, 5, 5, 5), (9, 300, 2, 2, 5, 5, 5)])
+def test_heff2_real_mpo(engine, prec, cplx, dims):
+    """Real (realified) MPO with real or complex vectors at shapes where the
+    small operator runs on MFMA (Hubbard-like d=4, w=6: K = 96)."""
+    import torch
+    Dl, Dr, d1, d2, wl, wm, wr = dims
+    rng = np.random.default_rng(sum(dims) + prec + 7 * cplx)
+    L = rnd(rng, (Dl, wl, Dl), cplx)
+    R = rnd(rng, (Dr, wr, Dr), cplx)
+    W1 = rnd(rng, (wl, d1, d1, wm), False)
+    W2 = rnd(rng, (wm, d2, d2, wr), False)
+    th = rnd(rng, (Dl, d1, d2, Dr), cplx)
+    ref = odmrg.heff2_matvec(L, W1, W2, R, th.shape)(th.reshape(-1).astype(complex)).reshape(th.shape)
+    et = engine.env_dtype(prec)
+    desc = engine.heff_desc(dev(engine, L_eng(L), et), dev(engine, R_eng(R), et), W_eng(W1), W_eng(W2),
+                            Dl, Dr, d1, d2, int(cplx), prec)
+    x = dev(engine, th)
+    y = engine.zeros(x.shape)
+    engine.heff_apply(desc, x, y)
+    torch.cuda.synchronize()
+    err = relerr(host(y), ref if cplx else ref.real)
+    assert err < TOL[prec] * 10, err
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("model", ["hubbard", "heisenberg"])
+def test_heff2_model_mpo(engine, prec, cplx, model):
+    """Bulk MPO tensors of the automaton MPO (sparse: the pre-contracted W
+    pair of the Hubbard chain has 124 nonzeros out of 96 x 96), which run on
+    the sparse small-operator kernel; inner sizes not multiples of 64."""
+    import torch
+    from dmrgpy_amd.mpo import build_mpo
+    from dmrgpy_amd.multioperator import heisenberg_terms, hubbard_terms
+    if model == "hubbard":
+        mpo = build_mpo([1] * 8, hubbard_terms(8, 1.0, 4.0))
+    else:
+        mpo = build_mpo([2] * 8, heisenberg_terms(8))
+    We1, We2 = mpo.W[3], mpo.W[4]                       # engine layout [1, wl, wr, s_out, s_in]
+    W1 = We1[0].transpose(0, 3, 2, 1)                   # oracle [wl, s_in, s_out, wr]
+    W2 = We2[0].transpose(0, 3, 2, 1)
+    wl, d1, wm, d2, wr = W1.shape[0], W1.shape[1], W1.shape[3], W2.shape[1], W2.shape[3]
+    Dl, Dr = 45, 77
+    rng = np.random.default_rng(3 + prec + 5 * cplx)
+    L = rnd(rng, (Dl, wl, Dl), cplx)
+    R = rnd(rng, (Dr, wr, Dr), cplx)
+    th = rnd(rng, (Dl, d1, d2, Dr), cplx)
+    ref = odmrg.heff2_matvec(L, W1, W2, R, th.shape)(th.reshape(-1).astype(complex)).reshape(th.shape)
+    et = engine.env_dtype(prec)
+    desc = engine.heff_desc(dev(engine, L_eng(L), et), dev(engine, R_eng(R), et), We1, We2,
+                            Dl, Dr, d1, d2, int(cplx), prec)
+    x = dev(engine, th)
+    y = engine.zeros(x.shape)
+    engine.heff_apply(desc, x, y)
+    torch.cuda.synchronize()
+    err = relerr(host(y), ref if cplx else ref.real)
+    assert err < TOL[prec] * 10, err
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("cplx", [False, True])
 def test_heff1(engine, prec, cplx):
