@@ -199,14 +199,21 @@ template <typename T, int TM, int TN, bool TA, bool TB, int TAG>
 __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const T* __restrict__ B,
                                                  T* __restrict__ C, GemmArgs g) {
   constexpr int BM = 32 * TM, BN = 32 * TN, BK = 16;
-  constexpr int PAD = (sizeof(T) == 8) ? 2 : 4;     // keeps rows 16-byte aligned
+  constexpr int PAD = 16 / sizeof(T);     // keeps rows 16-byte aligned
   constexpr int VW = Vec<T>::W;
   typedef typename Vec<T>::type vec_t;
   constexpr int NA = BM * BK / (VW * 256), NB = BN * BK / (VW * 256);
   typedef typename Mfma<T>::acc_t acc_t;
 
-  __shared__ __attribute__((aligned(16))) T As[2][BM][BK + PAD];
-  __shared__ __attribute__((aligned(16))) T Bs[2][BN][BK + PAD];
+  // Each operand tile keeps the orientation of its global rows, so every
+  // 16-byte global vector lands in LDS with one 16-byte store (no transposed
+  // scalar stores and their bank conflicts): [m][k] / [n][k] when k is the
+  // contiguous global index, [k][m] / [k][n] otherwise.  MFMA fragments read
+  // 4 consecutive k either as one vector or as 4 scalars from 4 rows.
+  constexpr int AR = TA ? BK : BM, AC = (TA ? BM : BK) + PAD;
+  constexpr int BR = TB ? BN : BK, BC = (TB ? BK : BN) + PAD;
+  __shared__ __attribute__((aligned(16))) T As[2][AR][AC];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BR][BC];
 
   const long long bz = blockIdx.z;
   A += bz * g.sA;
@@ -286,23 +293,21 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
     for (int e = 0; e < NA; ++e) {
       const int v = tid + 256 * e;
       if (!TA) {
-        const int r = v / (BK / VW), c = (v % (BK / VW)) * VW;
+        const int r = v / (BK / VW), c = (v % (BK / VW)) * VW;     // As[m][k]
         *reinterpret_cast<vec_t*>(&As[buf][r][c]) = ra[e];
       } else {
-        const int c = v / (BM / VW), r = (v % (BM / VW)) * VW;
-#pragma unroll
-        for (int j = 0; j < VW; ++j) As[buf][r + j][c] = ra[e][j];
+        const int c = v / (BM / VW), r = (v % (BM / VW)) * VW;     // As[k][m]
+        *reinterpret_cast<vec_t*>(&As[buf][c][r]) = ra[e];
       }
     }
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
       const int v = tid + 256 * e;
       if (!TB) {
-        const int c = v / (BN / VW), r = (v % (BN / VW)) * VW;
-#pragma unroll
-        for (int j = 0; j < VW; ++j) Bs[buf][r + j][c] = rb[e][j];
+        const int c = v / (BN / VW), r = (v % (BN / VW)) * VW;     // Bs[k][n]
+        *reinterpret_cast<vec_t*>(&Bs[buf][c][r]) = rb[e];
       } else {
-        const int r = v / (BK / VW), c = (v % (BK / VW)) * VW;
+        const int r = v / (BK / VW), c = (v % (BK / VW)) * VW;     // Bs[n][k]
         *reinterpret_cast<vec_t*>(&Bs[buf][r][c]) = rb[e];
       }
     }
@@ -317,13 +322,17 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
     if (more) load_regs(k0 + BK);
     T fa[TM][4], fb[TN][4];
 #pragma unroll
-    for (int a = 0; a < TM; ++a)
+    for (int a = 0; a < TM; ++a) {
+      const int row = wm * TM * 16 + a * 16 + li;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) fa[a][t] = As[cur][wm * TM * 16 + a * 16 + li][lk * 4 + t];
+      for (int t = 0; t < 4; ++t) fa[a][t] = TA ? As[cur][lk * 4 + t][row] : As[cur][row][lk * 4 + t];
+    }
 #pragma unroll
-    for (int b = 0; b < TN; ++b)
+    for (int b = 0; b < TN; ++b) {
+      const int col = wn * TN * 16 + b * 16 + li;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) fb[b][t] = Bs[cur][wn * TN * 16 + b * 16 + li][lk * 4 + t];
+      for (int t = 0; t < 4; ++t) fb[b][t] = TB ? Bs[cur][col][lk * 4 + t] : Bs[cur][lk * 4 + t][col];
+    }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
