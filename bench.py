@@ -41,6 +41,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def roctx_region():
+    """(resume, pause) of rocprofv3's selected-regions control (no-ops when
+    the ROCTx library is absent): the timed sweeps are the only region a
+    ``rocprofv3 --selected-regions`` run records."""
+    import ctypes
+    for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so", "libroctx64.so.4"):
+        try:
+            lib = ctypes.CDLL(name)
+            return (lambda: lib.roctxProfilerResume(0)), (lambda: lib.roctxProfilerPause(0))
+        except (OSError, AttributeError):
+            continue
+    return (lambda: None), (lambda: None)
+
+
+def calibrate_copy(nbytes, reps):
+    """Device-to-device copies of a known size, the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    resume, pause = roctx_region()
+    torch.cuda.synchronize()
+    resume()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    pause()
+    print(json.dumps({"calibration_copy_bytes": n * 8, "reps": reps}))
+
+
 CPU_SAMPLE = r"""
 import json, os, sys, time
 import numpy as np
@@ -85,7 +116,7 @@ def cpu_baseline(D, d, w, n_sites, iters_per_bond, threads):
             "sweep_s": sweep_s}
 
 
-def run_sweeps(ch, psi, K, prec_name, maxdim):
+def run_sweeps(ch, psi, K, prec_name, maxdim, region=False):
     import torch
     from dmrgpy_amd import algorithms as alg
     from dmrgpy_amd.chain import RES_TOL
@@ -94,12 +125,15 @@ def run_sweeps(ch, psi, K, prec_name, maxdim):
     prec = PREC_NAMES[prec_name]
     stats = alg.Stats()
     eng.prof_enable(True)
+    resume, pause = roctx_region() if region else ((lambda: None), (lambda: None))
     torch.cuda.synchronize()
+    resume()
     t0 = time.perf_counter()
     for _ in range(K):
         alg.dmrg_sweep(eng, psi, ch.H, maxdim, ch.cutoff, 2, prec, stats, res_tol=RES_TOL.get(prec, 0.0))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    pause()
     tms, flops, calls = eng.prof_read()
     eng.prof_enable(False)
     e = (alg.expectation(eng, psi, ch.H, psi) / alg.overlap(eng, psi, psi)).real
@@ -129,11 +163,16 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run device copies of this size (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (DMRG sweeps are sequential over bonds; "
               "see DESIGN.md for how H_eff would shard); --gpus {} is not supported".format(args.gpus))
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy, max(1, args.steps))
+        return
 
     import numpy as np
     from dmrgpy_amd import algorithms as alg
@@ -163,7 +202,7 @@ def main():
         log("warmup sweep {} D={} E={:.12f} maxbond={} t={:.1f}s".format(k, D, e, max(psi.bond_dims()),
                                                                         time.perf_counter() - t_setup))
     start = psi.copy()
-    res = run_sweeps(ch, psi, args.steps, args.precision, D)
+    res = run_sweeps(ch, psi, args.steps, args.precision, D, region=True)
     log("timed {}: {}".format(args.precision, res))
     res64 = None
     if not args.no_fp64:

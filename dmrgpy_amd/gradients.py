@@ -10,9 +10,13 @@ dE/dc_t = source_t * sum(G_s[wl_t, wr_t] * M_t).
 
 ``GroundStateEnergy`` wraps a DMRG solve as a ``torch.autograd.Function``
 whose backward applies these kernels, so couplings can be optimised with
-torch optimisers.  The reference has no gradient machinery; these values
-are checked against central finite differences and against ``vev`` of the
-bond operators (tests/test_gpu_gradients.py).
+torch optimisers; ``fit_bond_couplings`` uses it to recover couplings from
+target bond correlations.  ``tdvp_correlator_gradient`` differentiates a
+TDVP-evolved correlator with respect to every coupling by the adjoint
+method with checkpoints.  The reference has no gradient machinery; these
+values are checked against central finite differences, against ``vev`` of
+the bond operators and against exact dense time evolution
+(tests/test_gpu_gradients.py).
 """
 
 import numpy as np
@@ -105,3 +109,167 @@ def _autograd():
 def ground_state_energy(J, Delta, chain):
     """Differentiable E0(J, Delta) (torch tensors in, scalar tensor out)."""
     return _autograd().apply(J, Delta, chain)
+
+
+# ---------------------------------------------------------------------------
+# transition gradients <bra| dH/dc_t |ket>
+# ---------------------------------------------------------------------------
+def transition_mpo_gradients(eng, bra, ket, mpo, prec=0):
+    """[G_s]: d<bra|H|ket>/dW_s for every site (bra and ket may differ in
+    bond dimension), from mixed environments and the ``dmrg_mpo_grad``
+    kernel; host arrays [2, wl, wr, d, d]."""
+    bra = bra if bra.field else bra.as_complex()
+    ket = ket if ket.field else ket.as_complex()
+    n = ket.length()
+    L = {0: alg.ones_env(eng, 1, prec)}
+    for i in range(1, n):
+        L[i] = alg.env_left(eng, L[i - 1], ket.A(i), mpo.W[i - 1], 1, prec, bra=bra.A(i))
+    R = {n + 1: alg.ones_env(eng, 1, prec)}
+    for i in range(n, 1, -1):
+        R[i] = alg.env_right(eng, R[i + 1], ket.A(i), mpo.W[i - 1], 1, prec, bra=bra.A(i))
+    out = []
+    for s in range(1, n + 1):
+        W = mpo.W[s - 1]
+        out.append(eng.mpo_grad(1, prec, L[s - 1], bra.A(s), ket.A(s), R[s + 1], W.shape[1], W.shape[2]))
+    return out
+
+
+def transition_term_gradients(eng, bra, ket, mpo, nterms, prec=0):
+    """<bra| dH/dc_t |ket> for every input term t (complex array)."""
+    return contract_term_gradients(transition_mpo_gradients(eng, bra, ket, mpo, prec), mpo, nterms)
+
+
+# ---------------------------------------------------------------------------
+# adjoint TDVP
+# ---------------------------------------------------------------------------
+def simpson_weights(nt, dt):
+    """Composite Simpson weights on the grid s_k = k dt, k = 0..nt (nt even)."""
+    if nt % 2:
+        raise ValueError("Simpson's rule needs an even number of steps")
+    w = np.full(nt + 1, 2.0)
+    w[1::2] = 4.0
+    w[0] = w[-1] = 1.0
+    return w * dt / 3.0
+
+
+def tdvp_correlator_gradient(eng, codes, terms, psi1, psi2, e0, dt, nt, maxdim, cutoff=1e-12, niter=50,
+                             checkpoint=10, prec=0):
+    """C(T) = <psi2| psi(T)> and dC(T)/dc_t for every input term t of H.
+
+    psi(t) = exp(-i (H - e0) t) psi1 is evolved by the two-site TDVP steps of
+    quench_tdvp (src/dmrgpy/pyitensor/chain.py:520-542: shifted Hamiltonian,
+    renormalisation to the initial norm after each step; here from the
+    canonical gauge with the centre on site 1), T = nt dt.  With
+    psi1, psi2 and e0 held fixed,
+        dC/dc_t = -i int_0^T <phi(s)| dH/dc_t |psi(s)> ds,
+        phi(s)  = exp(+i (H - e0)(T - s)) psi2,
+    evaluated with Simpson's rule on the step grid.  phi runs backward from
+    T by TDVP steps of -dt; psi(s) is recomputed segment by segment from the
+    checkpoints stored every ``checkpoint`` steps of the forward pass, so at
+    most nt / checkpoint + checkpoint + 2 states are held at once.  The
+    integrand is the transition gradient <phi|dH/dc_t|psi> of the MPO
+    gradient kernel.  Returns (C(T), complex array over the terms)."""
+    from .mpo import build_mpo
+    nterm = len(terms)
+    Hs = build_mpo(codes, list(terms) + [(-e0, [("Id", 1)])], force_complex=False)
+    w = simpson_weights(nt, dt)
+    p1 = psi1.as_complex() if not psi1.field else psi1.copy()
+    p2 = psi2.as_complex() if not psi2.field else psi2.copy()
+    # orthogonality centre on site 1 before the first step: a first
+    # left-to-right half-sweep from another gauge (as quench_tdvp does right
+    # after the zip-up operator application, which leaves the centre on the
+    # last site) evolves with an H_eff that is no projection and makes an
+    # O(dt) error; from the canonical gauge the step is exact at full bond
+    # dimension and second order otherwise
+    alg.position(eng, p1, 1)
+    alg.position(eng, p2, 1)
+    n1 = np.sqrt(abs(alg.overlap(eng, p1, p1)))
+    n2 = np.sqrt(abs(alg.overlap(eng, p2, p2)))
+
+    def step(psi, d, norm):
+        alg.tdvp_step(eng, psi, Hs, d, cutoff, maxdim, niter, prec)
+        alg.normalize(eng, psi)
+        alg.scale_site(psi, psi.center or 1, norm)
+
+    ckpt = {0: p1.copy()}
+    psi = p1
+    for k in range(1, nt + 1):
+        step(psi, dt, n1)
+        if k % checkpoint == 0 or k == nt:
+            ckpt[k] = psi.copy()
+    C = alg.overlap(eng, p2, psi)
+    g = np.zeros(nterm + 1, dtype=complex)
+    phi = p2
+    marks = sorted(ckpt)
+    for seg in range(len(marks) - 2, -1, -1):
+        k0, k1 = marks[seg], marks[seg + 1]
+        states = {}
+        p = ckpt[k0].copy()
+        for k in range(k0 + 1, k1 + 1):
+            step(p, dt, n1)
+            states[k] = p.copy()
+        for k in range(k1, k0, -1):
+            g += w[k] * transition_term_gradients(eng, phi, states[k], Hs, nterm + 1, prec)
+            step(phi, -dt, n2)
+        del states
+    g += w[0] * transition_term_gradients(eng, phi, ckpt[0], Hs, nterm + 1, prec)
+    return C, -1j * g[:nterm]
+
+
+def xxz_correlator_gradients(eng, psi1, psi2, e0, J, Delta, dt, nt, maxdim, cutoff=1e-12, checkpoint=10, prec=0):
+    """(C(T), dC/dJ_i, dC/dDelta_i) for H = sum_i J_i (SxSx + SySy + Delta_i SzSz)."""
+    J = np.asarray(J, float)
+    Delta = np.asarray(Delta, float)
+    C, g = tdvp_correlator_gradient(eng, [2] * (len(J) + 1), xxz_terms(J, Delta), psi1, psi2, e0, dt, nt,
+                                    maxdim, cutoff, checkpoint=checkpoint, prec=prec)
+    gx, gy, gz = g[0::3], g[1::3], g[2::3]
+    return C, gx + gy + Delta * gz, J * gz
+
+
+# ---------------------------------------------------------------------------
+# fitting couplings to target bond correlations
+# ---------------------------------------------------------------------------
+def heisenberg_bond_terms(J):
+    t = []
+    for i, j in enumerate(np.asarray(J, float)):
+        for op in ("Sx", "Sy", "Sz"):
+            t.append((complex(j), [(op, i + 1), (op, i + 2)]))
+    return t
+
+
+def fit_bond_couplings(chain, target, J0, steps=60, tol=1e-7, callback=None):
+    """Couplings J of H(J) = sum_i J_i S_i.S_{i+1} whose ground state has the
+    target bond correlations t_i = <S_i.S_{i+1}>.
+
+    E0(J) is concave (a minimum of functions linear in J) and, by
+    Hellmann-Feynman, dE0/dJ_i = <S_i.S_{i+1}>_J, so F(J) = E0(J) - J.t is
+    concave with gradient <S_i.S_{i+1}>_J - t_i: its maximiser reproduces
+    the targets.  E0 is homogeneous of degree one in J, so the overall scale
+    is pinned by the penalty (mean(J) - mean(J0))^2.  L-BFGS on -F with the
+    device-side MPO-gradient kernels; every evaluation warm-starts DMRG from
+    the previous ground state.  Returns (J, max |<S.S> - t|, history)."""
+    from scipy.optimize import minimize
+    from .mpo import build_mpo
+    target = np.asarray(target, float)
+    J0 = np.asarray(J0, float)
+    m0 = J0.mean()
+    n = len(J0)
+    hist = []
+
+    def fun(J):
+        chain.set_hamiltonian(heisenberg_bond_terms(J))
+        e = chain.gs_energy()
+        mpo = build_mpo(chain.site_types, heisenberg_bond_terms(J))
+        g = term_gradients(chain.eng, chain.gs_wavefunction(), mpo, 3 * n).real
+        corr = g[0::3] + g[1::3] + g[2::3]
+        res = corr - target
+        hist.append(float(np.abs(res).max()))
+        if callback is not None:
+            callback(len(hist), J, hist[-1])
+        pen = J.mean() - m0
+        return -(e - J @ target) + pen ** 2, -res + 2.0 * pen / n
+
+    r = minimize(fun, J0, jac=True, method="L-BFGS-B",
+                 options={"maxiter": steps, "gtol": tol, "ftol": 1e-15, "maxcor": 20})
+    fun(r.x)
+    return r.x, hist[-1], hist
