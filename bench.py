@@ -41,20 +41,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def roctx_region():
-    """(resume, pause) of rocprofv3's selected-regions control (no-ops when
-    the ROCTx library is absent): the timed sweeps are the only region a
-    ``rocprofv3 --selected-regions`` run records."""
-    import ctypes
-    for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so", "libroctx64.so.4"):
-        try:
-            lib = ctypes.CDLL(name)
-            return (lambda: lib.roctxProfilerResume(0)), (lambda: lib.roctxProfilerPause(0))
-        except (OSError, AttributeError):
-            continue
-    return (lambda: None), (lambda: None)
-
-
 def calibrate_copy(nbytes, reps):
     """Device-to-device copies of a known size, the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -62,13 +48,10 @@ def calibrate_copy(nbytes, reps):
     n = nbytes // 8
     a = torch.ones(n, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
-    resume, pause = roctx_region()
     torch.cuda.synchronize()
-    resume()
     for _ in range(reps):
         b.copy_(a)
     torch.cuda.synchronize()
-    pause()
     print(json.dumps({"calibration_copy_bytes": n * 8, "reps": reps}))
 
 
@@ -116,7 +99,7 @@ def cpu_baseline(D, d, w, n_sites, iters_per_bond, threads):
             "sweep_s": sweep_s}
 
 
-def run_sweeps(ch, psi, K, prec_name, maxdim, region=False):
+def run_sweeps(ch, psi, K, prec_name, maxdim):
     import torch
     from dmrgpy_amd import algorithms as alg
     from dmrgpy_amd.chain import RES_TOL
@@ -125,22 +108,163 @@ def run_sweeps(ch, psi, K, prec_name, maxdim, region=False):
     prec = PREC_NAMES[prec_name]
     stats = alg.Stats()
     eng.prof_enable(True)
-    resume, pause = roctx_region() if region else ((lambda: None), (lambda: None))
     torch.cuda.synchronize()
-    resume()
     t0 = time.perf_counter()
     for _ in range(K):
         alg.dmrg_sweep(eng, psi, ch.H, maxdim, ch.cutoff, 2, prec, stats, res_tol=RES_TOL.get(prec, 0.0))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    pause()
     tms, flops, calls = eng.prof_read()
+    nbytes = eng.prof_read_bytes()
     eng.prof_enable(False)
     e = (alg.expectation(eng, psi, ch.H, psi) / alg.overlap(eng, psi, psi)).real
     return {"sweep_s": dt / K, "energy": e, "heff_ms": tms[0], "heff_flops": flops[0], "heff_calls": calls[0],
+            "heff_bytes": nbytes[0],
             "env_ms": tms[1], "svd_ms": tms[2], "svd_calls": calls[2],
             "iters_per_bond": sum(stats.lanczos_iters) / max(1, len(stats.lanczos_iters)),
             "max_bond": max(psi.bond_dims())}
+
+
+def measured_traffic(model, D, precision):
+    """HBM bytes per H_eff apply of this workload's timed sweep, as measured
+    with rocprofv3 FETCH_SIZE / WRITE_SIZE counters (calibrated against a
+    device copy of known size) and committed under profiles/; None when no
+    measurement of this exact workload exists."""
+    path = os.path.join(ROOT, "profiles", "{}_D{}_{}_heff_traffic.json".format(model, D, precision))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes_per_apply"]
+
+
+def workload_tdvp(args):
+    """Two-site TDVP after a local quench (configuration 5, sizes from the
+    flags): S=1 Heisenberg chain, DMRG ground state (FP32 ramp, final FP64
+    sweep), psi1 = psi2 = S^z_{N/2} |psi0>, quench_tdvp's steps of dt = 0.05 on
+    H - E0 with Krylov dimension 20 and bond dimension cap D, FP64
+    contractions.  Times `steps` TDVP steps after `warmup` untimed ones."""
+    import numpy as np
+    import torch
+    from dmrgpy_amd import algorithms as alg
+    from dmrgpy_amd.chain import Chain, RES_TOL
+    from dmrgpy_amd.mpo import build_mpo
+    from dmrgpy_amd.multioperator import heisenberg_terms
+    n = args.n or 128
+    D = args.D
+    ch = Chain([3] * n, field="real")
+    terms = heisenberg_terms(n)
+    ch.set_hamiltonian(terms)
+    eng = ch.eng
+    np.random.seed(0)
+    psi = ch._default_mps(maxm=64)
+    t0 = time.perf_counter()
+    for m in [m for m in (64, 128, 256, 512) if m < D] + [D, D]:
+        e = alg.dmrg_sweep(eng, psi, ch.H, m, 1e-12, 2, 1, None, res_tol=RES_TOL[1])
+        log("ground state ramp D={} E={:.10f} maxbond={} t={:.1f}s".format(m, e, max(psi.bond_dims()),
+                                                                       time.perf_counter() - t0))
+    e = alg.dmrg_sweep(eng, psi, ch.H, D, 1e-12, 2, 0, None)
+    e0 = (alg.expectation(eng, psi, ch.H, psi) / alg.overlap(eng, psi, psi)).real
+    Hs = build_mpo(ch.site_types, list(terms) + [(-e0, [("Id", 1)])])
+    A = build_mpo(ch.site_types, [(1.0, [("Sz", n // 2)])])
+    psi1 = alg.apply_mpo(eng, A, psi, 1e-12, D)
+    psi2 = psi1.copy()
+    alg.position(eng, psi1, 1)
+    norm0 = float(abs(alg.overlap(eng, psi1, psi1))) ** 0.5
+    dt = 0.05
+    stats = alg.Stats()
+
+    def step():
+        alg.tdvp_step(eng, psi1, Hs, dt, 1e-12, D, 20, 0, stats)
+        alg.normalize(eng, psi1)
+        alg.scale_site(psi1, psi1.center or 1, norm0)
+
+    corr = []
+    for _ in range(args.warmup):
+        corr.append(alg.overlap(eng, psi2, psi1))
+        step()
+    eng.prof_enable(True)
+    stats.reset()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        corr.append(alg.overlap(eng, psi2, psi1))
+        step()
+    torch.cuda.synchronize()
+    dt_s = (time.perf_counter() - ts) / args.steps
+    tms, flops, calls = eng.prof_read()
+    eng.prof_enable(False)
+    out = {"metric": "two-site TDVP step time after a local quench", "value": 1.0 / dt_s,
+           "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": dt_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp64", "data": "synthetic: DMRG ground state of the model, seeded random initial MPS",
+           "config": {"workload": "TDVP quench, open S=1 Heisenberg chain N={}, d=3, w=5, D cap {}, dt=0.05, "
+                                  "Krylov dim 20, Sz at site {}".format(n, D, n // 2),
+                      "N": n, "D": D, "E0": e0},
+           "max_bond": max(psi1.bond_dims()),
+           "krylov_iters_per_call": sum(stats.krylov_iters) / max(1, len(stats.krylov_iters)),
+           "time_split_s_per_step": {"heff": tms[0] / 1e3 / args.steps, "env": tms[1] / 1e3 / args.steps,
+                                     "svd": tms[2] / 1e3 / args.steps},
+           "heff_tflops": flops[0] / (tms[0] * 1e-3) / 1e12 if tms[0] > 0 else 0.0,
+           "correlator": [[c.real, c.imag] for c in corr],
+           "projected_200_steps_s": 200 * dt_s}
+    print(json.dumps(out))
+
+
+def workload_gradients(args):
+    """Coupling gradients (configuration 4, sizes from the flags): random-bond
+    XXZ chain, J_i, Delta_i ~ U[0.5, 1.5] from default_rng(seed 4), D cap,
+    FP32 contractions with a final FP64 sweep; all dE0/dJ_i, dE0/dDelta_i
+    from one backward pass of the MPO-gradient kernels, two couplings checked
+    by central finite differences of warm-started DMRG energies."""
+    import numpy as np
+    import torch
+    from dmrgpy_amd.chain import Chain
+    from dmrgpy_amd.gradients import xxz_gradients, xxz_terms
+    n = args.n or 64
+    D = args.D
+    rng = np.random.default_rng(4)
+    J = rng.uniform(0.5, 1.5, n - 1)
+    Dz = rng.uniform(0.5, 1.5, n - 1)
+
+    def solve(Jx, Dx, wf=None, nsw=8):
+        ch = Chain([2] * n, precision="fp32", field="real")
+        ch.set_sweep_params(D, nsw, 1e-12, 0.0)
+        ch.precision_schedule = ["fp32"] * (nsw - 1) + ["fp64"]
+        ch.set_hamiltonian(xxz_terms(Jx, Dx))
+        if wf is not None:
+            ch.set_wavefunction(wf.copy())
+        else:
+            np.random.seed(1)
+        return ch, ch.gs_energy()
+
+    t0 = time.perf_counter()
+    ch, e = solve(J, Dz)
+    t_gs = time.perf_counter() - t0
+    psi = ch.gs_wavefunction()
+    xxz_gradients(ch.eng, psi, J, Dz)       # warm-up (workspace growth)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gJ, gD = xxz_gradients(ch.eng, psi, J, Dz)
+    torch.cuda.synchronize()
+    t_grad = (time.perf_counter() - t0) / args.steps
+    h = 1e-4
+    checks = []
+    for i in (0, n // 2):
+        Jp, Jm = J.copy(), J.copy()
+        Jp[i] += h
+        Jm[i] -= h
+        fd = (solve(Jp, Dz, psi, 4)[1] - solve(Jm, Dz, psi, 4)[1]) / (2 * h)
+        checks.append({"coupling": "J_{}".format(i + 1), "grad": gJ[i], "fd": fd,
+                       "rel_err": abs(fd - gJ[i]) / abs(gJ[i])})
+    out = {"metric": "dE0/dJ and dE0/dDelta for every coupling (backward MPO-gradient kernels)",
+           "value": 1.0 / t_grad, "unit": "gradient passes/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": 1, "ms_per_step": t_grad * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp64", "data": "synthetic: seeded random-bond XXZ couplings",
+           "config": {"workload": "random-bond XXZ chain N={}, D={}, {} couplings".format(n, D, 2 * (n - 1)),
+                      "N": n, "D": D},
+           "ground_state_s": t_gs, "energy": e, "fd_checks": checks}
+    print(json.dumps(out))
 
 
 MODELS = {
@@ -163,6 +287,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--workload", default="dmrg", choices=["dmrg", "tdvp", "gradients"],
+                    help="dmrg: the headline sweep benchmark; tdvp / gradients: the TDVP-quench and "
+                         "coupling-gradient configurations (sizes from --n/--D/--steps)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run device copies of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -172,6 +299,12 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy, max(1, args.steps))
+        return
+    if args.workload == "tdvp":
+        workload_tdvp(args)
+        return
+    if args.workload == "gradients":
+        workload_gradients(args)
         return
 
     import numpy as np
@@ -202,7 +335,7 @@ def main():
         log("warmup sweep {} D={} E={:.12f} maxbond={} t={:.1f}s".format(k, D, e, max(psi.bond_dims()),
                                                                         time.perf_counter() - t_setup))
     start = psi.copy()
-    res = run_sweeps(ch, psi, args.steps, args.precision, D, region=True)
+    res = run_sweeps(ch, psi, args.steps, args.precision, D)
     log("timed {}: {}".format(args.precision, res))
     res64 = None
     if not args.no_fp64:
@@ -219,6 +352,7 @@ def main():
         cpu = cpu_baseline(D, spec["d"], spec["w"], n, res["iters_per_bond"], args.cpu_threads)
         log("cpu baseline: {}".format(cpu))
     achieved = flops_per / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    traffic = measured_traffic(args.model, D, args.precision)
     out = {
         "metric": METRIC,
         "value": 1.0 / res["sweep_s"],
@@ -247,7 +381,8 @@ def main():
                                    "svd": res["svd_ms"] / 1e3 / args.steps},
         "energy": res["energy"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
+                     "algorithmic_bytes": res["heff_bytes"] / max(1, res["heff_calls"]),
                      "kernel": "H_eff two-site apply (L-GEMM, W small-op, R-GEMM), {:.3g} GFLOP per apply "
                                "averaged over the sweep's bonds, {:.3f} ms average".format(flops_per / 1e9, avg_ms)},
     }

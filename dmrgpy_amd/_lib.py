@@ -57,6 +57,7 @@ SIGNATURES = {
                           c_double, c_void_p, c_int, c_ll, c_ll, c_int]),
     "dmrg_prof_enable": (c_int, [c_void_p, c_int]),
     "dmrg_prof_read": (c_int, [c_void_p, P_double, P_double, P_ll]),
+    "dmrg_prof_read_bytes": (c_int, [c_void_p, P_double]),
     "dmrg_matvec_count": (c_ll, [c_void_p]),
 }
 

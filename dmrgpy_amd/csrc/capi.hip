@@ -49,6 +49,7 @@ static void prof_drain(Ctx* c, bool all) {
     DMRG_CHECK_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
     c->prof_time_ms[p.kind] += ms;
     c->prof_flops[p.kind] += p.flops;
+    c->prof_bytes[p.kind] += p.bytes;
     c->prof_calls[p.kind] += 1;
     c->event_pool.push_back(p.start);
     c->event_pool.push_back(p.stop);
@@ -56,7 +57,8 @@ static void prof_drain(Ctx* c, bool all) {
   c->prof.swap(keep);
 }
 
-ProfScope::ProfScope(Ctx* c_, int kind_, double flops_) : c(c_), kind(kind_), flops(flops_) {
+ProfScope::ProfScope(Ctx* c_, int kind_, double flops_, double bytes_)
+    : c(c_), kind(kind_), flops(flops_), bytes(bytes_) {
   if (!c->prof_on) return;
   a = pool_event(c);
   b = pool_event(c);
@@ -65,7 +67,7 @@ ProfScope::ProfScope(Ctx* c_, int kind_, double flops_) : c(c_), kind(kind_), fl
 ProfScope::~ProfScope() {
   if (!a) return;
   if (hipEventRecord(b, c->stream) != hipSuccess) return;
-  c->prof.push_back(ProfEntry{a, b, flops, kind});
+  c->prof.push_back(ProfEntry{a, b, flops, bytes, kind});
   if (c->prof.size() > 4096) {
     try {
       prof_drain(c, false);
@@ -239,6 +241,7 @@ int dmrg_prof_enable(dmrg_ctx* ctx, int on) {
   for (int i = 0; i < 4; ++i) {
     c.prof_time_ms[i] = 0;
     c.prof_flops[i] = 0;
+    c.prof_bytes[i] = 0;
     c.prof_calls[i] = 0;
   }
   c.prof_on = on != 0;
@@ -254,6 +257,15 @@ int dmrg_prof_read(dmrg_ctx* ctx, double* time_ms, double* flops, long long* cal
     if (flops) flops[i] = c.prof_flops[i];
     if (calls) calls[i] = c.prof_calls[i];
   }
+  API_END
+}
+
+int dmrg_prof_read_bytes(dmrg_ctx* ctx, double* bytes) {
+  API_BEGIN
+  dmrg::Ctx& c = ctx->c;
+  dmrg::prof_drain(&c, true);
+  for (int i = 0; i < 4; ++i)
+    if (bytes) bytes[i] = c.prof_bytes[i];
   API_END
 }
 

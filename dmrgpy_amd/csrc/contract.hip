@@ -514,6 +514,15 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   // offsets in the small-op tables are 32-bit
   DMRG_REQUIRE((long long)wl * Dl * h.P * Dr < (1LL << 31), "heff: tensor too large for 32-bit offsets");
   h.flops_per_apply = heff_flops(d);
+  {
+    const double np_ = d.field == CPLX ? 2.0 : 1.0, es = (double)prec_size(d.prec);
+    const double n = (double)h.n, t1 = (double)wl * Dl * h.P * Dr, t2 = (double)Dl * h.P * wr * Dr;
+    double b = np_ * es * ((double)wl * Dl * Dl + n + t1     // L-GEMM: L, x in, T1 out
+                           + t1 + t2                          // small operator: T1 in, T2 out
+                           + t2 + (double)wr * Dr * Dr + n);  // R-GEMM: T2, R in, y out
+    if (d.prec != P64) b += np_ * n * (8.0 + es) * 2.0;       // x to and y from FP64
+    h.bytes_per_apply = b;
+  }
   return h;
 }
 
@@ -540,7 +549,7 @@ void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
   const long long n = h.n;
   const long long nT = (long long)wl * Dl * P * Dr;   // T1 per plane
   const long long nT2 = (long long)Dl * P * wr * Dr;  // T2 per plane
-  ProfScope ps(c, 0, h.flops_per_apply);
+  ProfScope ps(c, 0, h.flops_per_apply, h.bytes_per_apply);
   const void* xin = x;
   void* yout = y;
   if (d.prec != P64) {

@@ -80,6 +80,7 @@ enum Slot {
 struct ProfEntry {
   hipEvent_t start, stop;
   double flops;
+  double bytes;
   int kind;
 };
 
@@ -95,6 +96,7 @@ struct Ctx {
   std::vector<hipEvent_t> event_pool;
   double prof_time_ms[4] = {0, 0, 0, 0};
   double prof_flops[4] = {0, 0, 0, 0};
+  double prof_bytes[4] = {0, 0, 0, 0};   // algorithmic HBM bytes
   long long prof_calls[4] = {0, 0, 0, 0};
   long long matvecs = 0;
 };

@@ -30,6 +30,7 @@ struct HeffOp {
   int P = 1;            // product of the physical dims acted on
   long long n = 0;      // vector length per plane
   double flops_per_apply = 0;
+  double bytes_per_apply = 0;   // algorithmic HBM bytes (each operand read once, each result written once)
 };
 
 // Records a pair of HIP events around a region on the context stream when
@@ -38,9 +39,9 @@ struct HeffOp {
 struct ProfScope {
   Ctx* c;
   int kind;
-  double flops;
+  double flops, bytes;
   hipEvent_t a = nullptr, b = nullptr;
-  ProfScope(Ctx* c_, int kind_, double flops_);
+  ProfScope(Ctx* c_, int kind_, double flops_, double bytes_ = 0.0);
   ~ProfScope();
 };
 

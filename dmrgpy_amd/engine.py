@@ -185,6 +185,11 @@ class Engine:
         check(self.lib.dmrg_prof_read(self.ctx, t, f, n), "dmrg_prof_read")
         return list(t), list(f), list(n)
 
+    def prof_read_bytes(self):
+        b = (ctypes.c_double * 4)()
+        check(self.lib.dmrg_prof_read_bytes(self.ctx, b), "dmrg_prof_read_bytes")
+        return list(b)
+
     def matvec_count(self):
         return int(self.lib.dmrg_matvec_count(self.ctx))
 

@@ -142,6 +142,10 @@ int dmrg_gemm(dmrg_ctx* ctx, int prec, int field, int transA, int transB, int co
  * HIP events around each call on the context stream. */
 int dmrg_prof_enable(dmrg_ctx* ctx, int on);
 int dmrg_prof_read(dmrg_ctx* ctx, double* time_ms, double* flops, long long* calls);
+/* Algorithmic HBM bytes of the profiled calls per slot (H_eff: every operand
+ * read once and every intermediate written once; the roofline denominator
+ * of the measured traffic).  No reference counterpart (measurement only). */
+int dmrg_prof_read_bytes(dmrg_ctx* ctx, double* bytes);
 /* Total H_eff applies issued through the context. */
 long long dmrg_matvec_count(dmrg_ctx* ctx);
 
