@@ -23,7 +23,8 @@ class HeffDesc(ctypes.Structure):
                 ("Dl", c_int), ("Dr", c_int), ("d1", c_int), ("d2", c_int),
                 ("wl", c_int), ("wm", c_int), ("wr", c_int),
                 ("L", c_void_p), ("R", c_void_p),
-                ("W1", c_void_p), ("W2", c_void_p), ("w_complex", c_int)]
+                ("W1", c_void_p), ("W2", c_void_p), ("w_complex", c_int),
+                ("ldL", c_int), ("ldR", c_int)]
 
 
 class EnvDesc(ctypes.Structure):
@@ -31,7 +32,8 @@ class EnvDesc(ctypes.Structure):
                 ("Dl", c_int), ("Dr", c_int), ("Dlb", c_int), ("Drb", c_int),
                 ("d", c_int), ("wl", c_int), ("wr", c_int),
                 ("ket", c_void_p), ("bra", c_void_p), ("W", c_void_p), ("w_complex", c_int),
-                ("env_in", c_void_p), ("env_out", c_void_p)]
+                ("env_in", c_void_p), ("env_out", c_void_p),
+                ("ld_in", c_int), ("ld_out", c_int)]
 
 
 # name -> (restype, argtypes); every symbol include/dmrg_hip.h declares
@@ -51,7 +53,7 @@ SIGNATURES = {
     "dmrg_svd_split": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_int, c_int, c_int,
                                c_void_p, c_void_p, c_int, P_int, P_double, P_double]),
     "dmrg_mpo_grad": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
-                              c_int, c_int, c_int, c_int, c_int, c_void_p]),
+                              c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p]),
     "dmrg_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                           c_double, c_void_p, c_int, c_ll, c_ll, c_void_p, c_int, c_ll, c_ll,
                           c_double, c_void_p, c_int, c_ll, c_ll, c_int]),

@@ -61,15 +61,18 @@ def theta2(eng, psi, i):
 
 
 def env_left(eng, L, A, W, field, prec, bra=None):
+    """L[w][D_bra][D_ket] one site further right; the last index is stored
+    with a row pitch rounded up to a multiple of 4 (Engine.env_pitch)."""
     bra = A if bra is None else bra
-    out = eng.empty((L.shape[0], W.shape[2], bra.shape[3], A.shape[3]), dtype=L.dtype)
+    out = eng.empty((L.shape[0], W.shape[2], bra.shape[3], eng.env_pitch(A.shape[3])), dtype=L.dtype)
     eng.env_update("left", field, prec, A, bra, W, L, out)
     return out
 
 
 def env_right(eng, R, A, W, field, prec, bra=None):
+    """R[w][D_ket][D_bra] one site further left (last index pitched like env_left)."""
     bra = A if bra is None else bra
-    out = eng.empty((R.shape[0], W.shape[1], A.shape[1], bra.shape[1]), dtype=R.dtype)
+    out = eng.empty((R.shape[0], W.shape[1], A.shape[1], eng.env_pitch(bra.shape[1])), dtype=R.dtype)
     eng.env_update("right", field, prec, A, bra, W, R, out)
     return out
 

@@ -20,7 +20,7 @@ double rayleigh(Ctx* c, const HeffOp& h, const double* x);
 int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
                     int absorb, double* U, double* V, int kcap, int* keep, double* discarded, double* svals_host);
 void mpo_grad(Ctx* c, int field, int prec, const void* L, const double* bra64, const double* ket64, const void* R,
-              int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, double* G_host);
+              int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, int ldL, int ldR, double* G_host);
 
 void* ws_get(Ctx* c, Slot s, size_t bytes) { return c->ws[s].get(bytes, c->stream); }
 
@@ -203,13 +203,17 @@ int dmrg_svd_split(dmrg_ctx* ctx, int field, const double* M, int m, int n, doub
 }
 
 int dmrg_mpo_grad(dmrg_ctx* ctx, int field, int prec, const void* L, const double* bra, const double* ket,
-                  const void* R, int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, double* G_host) {
+                  const void* R, int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, int ldL, int ldR,
+                  double* G_host) {
   API_BEGIN
   DMRG_REQUIRE(ctx && L && bra && ket && R && G_host, "mpo_grad: null argument");
   DMRG_REQUIRE(field == 0 || field == 1, "mpo_grad: bad field");
   DMRG_REQUIRE(prec >= 0 && prec <= 2, "mpo_grad: bad precision");
   DMRG_REQUIRE(Dl > 0 && Dr > 0 && Dlb > 0 && Drb > 0 && d > 0 && wl > 0 && wr > 0, "mpo_grad: bad dims");
-  dmrg::mpo_grad(&ctx->c, field, prec, L, bra, ket, R, Dl, Dr, Dlb, Drb, d, wl, wr, G_host);
+  if (ldL == 0) ldL = Dl;
+  if (ldR == 0) ldR = Drb;
+  DMRG_REQUIRE(ldL >= Dl && ldR >= Drb, "mpo_grad: row pitch smaller than the dimension");
+  dmrg::mpo_grad(&ctx->c, field, prec, L, bra, ket, R, Dl, Dr, Dlb, Drb, d, wl, wr, ldL, ldR, G_host);
   API_END
 }
 

@@ -116,6 +116,34 @@ void from_prec(Ctx* c, int prec, const void* src, double* dst, long long n) {
 
 size_t prec_size(int prec) { return prec == P64 ? sizeof(double) : sizeof(float); }
 
+// dst[pl][r][0..cols) (row pitch ld) = src[pl][r][0..cols) (dense), converted
+// to T; planes of rows*cols (src) and rows*ld (dst) elements
+template <typename T>
+__global__ void convert_pitched_k(const double* __restrict__ src, T* __restrict__ dst, long long rows, int cols,
+                                  int ld, int planes) {
+  const long long tot = (long long)planes * rows * cols;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < tot; i += stride) {
+    const long long pr = i / cols;   // plane * rows + row
+    const int col = (int)(i - pr * cols);
+    dst[pr * ld + col] = (T)src[i];
+  }
+}
+
+void to_prec_pitched(Ctx* c, int prec, const double* src, void* dst, long long rows, int cols, int ld, int planes) {
+  if (ld == cols) {
+    to_prec(c, prec, src, dst, rows * cols * planes);
+    return;
+  }
+  const long long tot = (long long)planes * rows * cols;
+  if (prec == P64)
+    convert_pitched_k<double><<<grid1d(tot), 256, 0, c->stream>>>(src, (double*)dst, rows, cols, ld, planes);
+  else
+    convert_pitched_k<float><<<grid1d(tot), 256, 0, c->stream>>>(src, (float*)dst, rows, cols, ld, planes);
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
 // Uploads a small operator (W rows padded to a multiple of OCH_PAD), its
 // offset tables and, for a real W, its compressed rows into a workspace slot;
 // returns device pointers.
@@ -447,6 +475,12 @@ static double wget(const double* W, int i0, int i1, int i2, int i3, int n1, int 
   return W[(((size_t)i0 * n1 + i1) * n2 + i2) * n3 + i3];
 }
 
+// Row pitches that keep every H_eff GEMM operand row 16-byte aligned: the
+// T2 intermediate [(a' p')][(wr b)] and, when P*Dr is not a multiple of 4,
+// the contraction-precision copy of theta.
+static int pad4(long long v) { return (int)((v + 3) / 4 * 4); }
+static int t2_pitch(int wr, int Dr) { return pad4((long long)wr * Dr); }
+
 HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   HeffOp h;
   h.d = d;
@@ -455,6 +489,8 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   DMRG_REQUIRE(d.field == REAL || d.field == CPLX, "heff: bad field");
   DMRG_REQUIRE(d.prec == P64 || d.prec == P32 || d.prec == PB3, "heff: bad precision");
   DMRG_REQUIRE(!(d.w_complex && d.field == REAL), "heff: complex MPO requires complex field");
+  DMRG_REQUIRE((d.ldL == 0 || d.ldL >= d.Dl) && (d.ldR == 0 || d.ldR >= d.Dr),
+               "heff: environment row pitch smaller than the dimension");
   const int Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr;
   const bool wc = d.w_complex != 0;
   if (d.nsite == 2) {
@@ -486,7 +522,7 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
     for (int a = 0; a < wl; ++a)
       for (int p = 0; p < P; ++p) ioff[a * P + p] = a * Dl * P * Dr + p * Dr;
     for (int p = 0; p < P; ++p)
-      for (int b = 0; b < wr; ++b) ooff[p * wr + b] = (p * wr + b) * Dr;
+      for (int b = 0; b < wr; ++b) ooff[p * wr + b] = p * t2_pitch(wr, Dr) + b * Dr;
     h.sop = upload_smallop(c, WS_WSMALL, d.prec, wre, wim, wc, Kin, Kout, ioff, ooff);
     h.P = P;
   } else {
@@ -506,13 +542,14 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
     for (int a = 0; a < wl; ++a)
       for (int p = 0; p < P; ++p) ioff[a * P + p] = a * Dl * P * Dr + p * Dr;
     for (int p = 0; p < P; ++p)
-      for (int b = 0; b < wr; ++b) ooff[p * wr + b] = (p * wr + b) * Dr;
+      for (int b = 0; b < wr; ++b) ooff[p * wr + b] = p * t2_pitch(wr, Dr) + b * Dr;
     h.sop = upload_smallop(c, WS_WSMALL, d.prec, wre, wim, wc, Kin, Kout, ioff, ooff);
     h.P = P;
   }
   h.n = (long long)Dl * h.P * Dr;
   // offsets in the small-op tables are 32-bit
-  DMRG_REQUIRE((long long)wl * Dl * h.P * Dr < (1LL << 31), "heff: tensor too large for 32-bit offsets");
+  DMRG_REQUIRE((long long)wl * Dl * h.P * Dr < (1LL << 31) && (long long)Dl * h.P * t2_pitch(wr, Dr) < (1LL << 31),
+               "heff: tensor too large for 32-bit offsets");
   h.flops_per_apply = heff_flops(d);
   {
     const double np_ = d.field == CPLX ? 2.0 : 1.0, es = (double)prec_size(d.prec);
@@ -544,31 +581,37 @@ double heff_flops(const dmrg_heff_desc& d) {
 void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
   const dmrg_heff_desc& d = h.d;
   const int Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr, P = h.P;
+  const int ldL = d.ldL ? d.ldL : Dl, ldR = d.ldR ? d.ldR : Dr;
+  const int ldT2 = t2_pitch(wr, Dr);
   const int np = d.field == CPLX ? 2 : 1;
   const size_t es = prec_size(d.prec);
   const long long n = h.n;
   const long long nT = (long long)wl * Dl * P * Dr;   // T1 per plane
-  const long long nT2 = (long long)Dl * P * wr * Dr;  // T2 per plane
+  const long long nT2 = (long long)Dl * P * ldT2;     // T2 per plane (rows pitched)
   ProfScope ps(c, 0, h.flops_per_apply, h.bytes_per_apply);
+  // theta as the L-GEMM operand: rows of P*Dr elements, pitched to a multiple
+  // of 4 when needed; converted to the contraction precision on the way
+  const int ldx = ((long long)P * Dr) % 4 == 0 ? P * Dr : pad4((long long)P * Dr);
+  const long long nx = (long long)Dl * ldx;
   const void* xin = x;
   void* yout = y;
-  if (d.prec != P64) {
-    void* xb = ws_get(c, WS_XIN, n * np * es);
-    to_prec(c, d.prec, x, xb, n * np);
+  if (d.prec != P64 || ldx != P * Dr) {
+    void* xb = ws_get(c, WS_XIN, nx * np * es);
+    to_prec_pitched(c, d.prec, x, xb, Dl, P * Dr, ldx, np);
     xin = xb;
-    yout = ws_get(c, WS_YOUT, n * np * es);
   }
+  if (d.prec != P64) yout = ws_get(c, WS_YOUT, n * np * es);
   void* T1 = ws_get(c, WS_T1, nT * np * es);
   void* T2 = ws_get(c, WS_T2, nT2 * np * es);
   {  // T1[(wl a')][(p b)] = L[(wl a')][a] * x[a][(p b)]
-    GemmDesc g = gdesc(wl * Dl, P * Dr, Dl, Dl, P * Dr, P * Dr, d.field, 1);
-    g.pA = (long long)wl * Dl * Dl; g.pB = n; g.pC = nT;
+    GemmDesc g = gdesc(wl * Dl, P * Dr, Dl, ldL, ldx, P * Dr, d.field, 1);
+    g.pA = (long long)wl * Dl * ldL; g.pB = nx; g.pC = nT;
     gemm_prec(c, d.prec, g, d.L, xin, T1);
   }
-  run_smallop(c, d.prec, d.field, h.sop, T1, T2, Dl, Dr, (long long)P * Dr, (long long)P * wr * Dr, nT, nT2);
+  run_smallop(c, d.prec, d.field, h.sop, T1, T2, Dl, Dr, (long long)P * Dr, (long long)P * ldT2, nT, nT2);
   {  // y[(a' p')][b'] = T2[(a' p')][(wr b)] * R[(wr b)][b']
-    GemmDesc g = gdesc(Dl * P, Dr, wr * Dr, wr * Dr, Dr, Dr, d.field, 2);
-    g.pA = nT2; g.pB = (long long)wr * Dr * Dr; g.pC = n;
+    GemmDesc g = gdesc(Dl * P, Dr, wr * Dr, ldT2, ldR, Dr, d.field, 2);
+    g.pA = nT2; g.pB = (long long)wr * Dr * ldR; g.pC = n;
     gemm_prec(c, d.prec, g, T2, d.R, yout);
   }
   if (d.prec != P64) from_prec(c, d.prec, yout, y, n * np);
@@ -585,6 +628,11 @@ void env_update(Ctx* c, const dmrg_env_desc& e) {
   DMRG_REQUIRE(!(e.w_complex && e.field == REAL), "env: complex MPO requires complex field");
   const int Dl = e.Dl, Dr = e.Dr, Dlb = e.Dlb, Drb = e.Drb, dd = e.d, wl = e.wl, wr = e.wr;
   const int np = e.field == CPLX ? 2 : 1;
+  // row pitches of the environments' last index (dir 0: Dl in, Dr out; dir 1: Drb in, Dlb out)
+  const int ldIn = e.ld_in ? e.ld_in : (e.dir == 0 ? Dl : Drb);
+  const int ldOut = e.ld_out ? e.ld_out : (e.dir == 0 ? Dr : Dlb);
+  DMRG_REQUIRE(ldIn >= (e.dir == 0 ? Dl : Drb) && ldOut >= (e.dir == 0 ? Dr : Dlb),
+               "env: row pitch smaller than the dimension");
   const size_t es = prec_size(e.prec);
   const bool wc = e.w_complex != 0;
   const long long nk = (long long)Dl * dd * Dr, nb = (long long)Dlb * dd * Drb;
@@ -627,8 +675,8 @@ void env_update(Ctx* c, const dmrg_env_desc& e) {
     const long long n1 = (long long)wl * Dlb * dd * Dr, n2 = (long long)wr * Dlb * dd * Dr;
     void* E1 = ws_get(c, WS_ENV1, n1 * np * es);
     void* E2 = ws_get(c, WS_ENV2, n2 * np * es);
-    GemmDesc g = gdesc(wl * Dlb, dd * Dr, Dl, Dl, dd * Dr, dd * Dr, e.field, 3);
-    g.pA = (long long)wl * Dlb * Dl; g.pB = nk; g.pC = n1;
+    GemmDesc g = gdesc(wl * Dlb, dd * Dr, Dl, ldIn, dd * Dr, dd * Dr, e.field, 3);
+    g.pA = (long long)wl * Dlb * ldIn; g.pB = nk; g.pC = n1;
     gemm_prec(c, e.prec, g, e.env_in, ket, E1);
     std::vector<int> ioff(Kin), ooff(Kout);
     for (int a = 0; a < wl; ++a)
@@ -638,21 +686,21 @@ void env_update(Ctx* c, const dmrg_env_desc& e) {
     SmallOp so = upload_smallop(c, WS_WSMALL, e.prec, wre, wim, wc, Kin, Kout, ioff, ooff);
     run_smallop(c, e.prec, e.field, so, E1, E2, Dlb, Dr, (long long)dd * Dr, (long long)dd * Dr, n1, n2);
     // Lnew[wr][b'][b] = sum_{(a' s')} conj(bra[(a' s')][b']) E2[wr][(a' s')][b]
-    GemmDesc h = gdesc(Drb, Dr, Dlb * dd, Drb, Dr, Dr, e.field, 3);
+    GemmDesc h = gdesc(Drb, Dr, Dlb * dd, Drb, Dr, ldOut, e.field, 3);
     h.transA = true; h.conjA = true;
     h.batch = wr;
-    h.sA = 0; h.sB = (long long)Dlb * dd * Dr; h.sC = (long long)Drb * Dr;
-    h.pA = nb; h.pB = n2; h.pC = (long long)wr * Drb * Dr;
+    h.sA = 0; h.sB = (long long)Dlb * dd * Dr; h.sC = (long long)Drb * ldOut;
+    h.pA = nb; h.pB = n2; h.pC = (long long)wr * Drb * ldOut;
     gemm_prec(c, e.prec, h, bra, E2, e.env_out);
   } else {
     // F1[wr][(a s)][b'] = ket[(a s)][b] * R[wr][b][b']
     const long long n1 = (long long)wr * Dl * dd * Drb, n2 = (long long)wl * Dl * dd * Drb;
     void* F1 = ws_get(c, WS_ENV1, n1 * np * es);
     void* F2 = ws_get(c, WS_ENV2, n2 * np * es);
-    GemmDesc g = gdesc(Dl * dd, Drb, Dr, Dr, Drb, Drb, e.field, 3);
+    GemmDesc g = gdesc(Dl * dd, Drb, Dr, Dr, ldIn, Drb, e.field, 3);
     g.batch = wr;
-    g.sA = 0; g.sB = (long long)Dr * Drb; g.sC = (long long)Dl * dd * Drb;
-    g.pA = nk; g.pB = (long long)wr * Dr * Drb; g.pC = n1;
+    g.sA = 0; g.sB = (long long)Dr * ldIn; g.sC = (long long)Dl * dd * Drb;
+    g.pA = nk; g.pB = (long long)wr * Dr * ldIn; g.pC = n1;
     gemm_prec(c, e.prec, g, ket, e.env_in, F1);
     std::vector<int> ioff(Kin), ooff(Kout);
     for (int b = 0; b < wr; ++b)
@@ -662,11 +710,11 @@ void env_update(Ctx* c, const dmrg_env_desc& e) {
     SmallOp so = upload_smallop(c, WS_WSMALL, e.prec, wre, wim, wc, Kin, Kout, ioff, ooff);
     run_smallop(c, e.prec, e.field, so, F1, F2, Dl, Drb, (long long)dd * Drb, (long long)dd * Drb, n1, n2);
     // Rnew[wl][a][a'] = sum_{(s' b')} F2[wl][a][(s' b')] conj(bra[a'][(s' b')])
-    GemmDesc h = gdesc(Dl, Dlb, dd * Drb, dd * Drb, dd * Drb, Dlb, e.field, 3);
+    GemmDesc h = gdesc(Dl, Dlb, dd * Drb, dd * Drb, dd * Drb, ldOut, e.field, 3);
     h.transB = true; h.conjB = true;
     h.batch = wl;
-    h.sA = (long long)Dl * dd * Drb; h.sB = 0; h.sC = (long long)Dl * Dlb;
-    h.pA = n2; h.pB = nb; h.pC = (long long)wl * Dl * Dlb;
+    h.sA = (long long)Dl * dd * Drb; h.sB = 0; h.sC = (long long)Dl * ldOut;
+    h.pA = n2; h.pB = nb; h.pC = (long long)wl * Dl * ldOut;
     gemm_prec(c, e.prec, h, F2, bra, e.env_out);
   }
 }

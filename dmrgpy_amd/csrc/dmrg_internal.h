@@ -48,6 +48,7 @@ struct ProfScope {
 size_t prec_size(int prec);
 void to_prec(Ctx* c, int prec, const double* src, void* dst, long long n);
 void from_prec(Ctx* c, int prec, const void* src, double* dst, long long n);
+void to_prec_pitched(Ctx* c, int prec, const double* src, void* dst, long long rows, int cols, int ld, int planes);
 SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& wre,
                        const std::vector<double>& wim, bool wcplx, int Kin, int Kout,
                        const std::vector<int>& ioff, const std::vector<int>& ooff);

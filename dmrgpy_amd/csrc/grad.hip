@@ -51,7 +51,7 @@ static unsigned gg(long long n) {
 }
 
 void mpo_grad(Ctx* c, int field, int prec, const void* L, const double* bra64, const double* ket64, const void* R,
-              int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, double* G_host) {
+              int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, int ldL, int ldR, double* G_host) {
   const int np = field == CPLX ? 2 : 1;
   const size_t es = prec_size(prec);
   const long long nk = (long long)Dl * d * Dr, nb = (long long)Dlb * d * Drb;
@@ -72,15 +72,15 @@ void mpo_grad(Ctx* c, int field, int prec, const void* L, const double* bra64, c
   void* X2p = ws_get(c, WS_ENV1, n2 * np * es);
   void* Bp = ws_get(c, WS_ENV2, nb * np * es);
   {
-    GemmDesc g = gdesc(wl * Dlb, d * Dr, Dl, Dl, d * Dr, d * Dr, field, 5);
-    g.pA = (long long)wl * Dlb * Dl; g.pB = nk; g.pC = n1;
+    GemmDesc g = gdesc(wl * Dlb, d * Dr, Dl, ldL, d * Dr, d * Dr, field, 5);
+    g.pA = (long long)wl * Dlb * ldL; g.pB = nk; g.pC = n1;
     gemm_prec(c, prec, g, L, ket, X1);
   }
   {
-    GemmDesc g = gdesc(wl * Dlb * d, Drb, Dr, Dr, Drb, Drb, field, 5);
+    GemmDesc g = gdesc(wl * Dlb * d, Drb, Dr, Dr, ldR, Drb, field, 5);
     g.batch = wr;
-    g.sA = 0; g.sB = (long long)Dr * Drb; g.sC = (long long)wl * Dlb * d * Drb;
-    g.pA = n1; g.pB = (long long)wr * Dr * Drb; g.pC = n2;
+    g.sA = 0; g.sB = (long long)Dr * ldR; g.sC = (long long)wl * Dlb * d * Drb;
+    g.pA = n1; g.pB = (long long)wr * Dr * ldR; g.pC = n2;
     gemm_prec(c, prec, g, X1, R, X2);
   }
   // X2 [(wr wl)][a'][s][b'] -> X2p [(wr wl)][s][a'][b'];  bra [a'][s'][b'] -> Bp [s'][a'][b']

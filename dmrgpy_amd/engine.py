@@ -61,6 +61,12 @@ class Engine:
     def zeros(self, shape, dtype=None):
         return self.torch.zeros(shape, dtype=dtype or self.torch.float64, device=self.device)
 
+    @staticmethod
+    def env_pitch(D):
+        """Row pitch for an environment's last index: a multiple of 4
+        elements, so the H_eff and environment GEMMs load 16-byte vectors."""
+        return (int(D) + 3) // 4 * 4
+
     def env_dtype(self, prec):
         return self.torch.float64 if prec == 0 else self.torch.float32
 
@@ -93,6 +99,7 @@ class Engine:
         d.W1 = _hptr(W1)
         d.L = _ptr(L)
         d.R = _ptr(R)
+        d.ldL, d.ldR = int(L.shape[-1]), int(R.shape[-1])   # row pitch of the last index
         d._hold = (W1, W2, L, R)
         return d
 
@@ -134,6 +141,7 @@ class Engine:
         e.wl, e.wr = W.shape[1], W.shape[2]
         e.ket, e.bra, e.W = _ptr(ket), _ptr(bra), _hptr(W)
         e.env_in, e.env_out = _ptr(env_in), _ptr(env_out)
+        e.ld_in, e.ld_out = int(env_in.shape[-1]), int(env_out.shape[-1])
         check(self.lib.dmrg_env_update(self.ctx, ctypes.byref(e)), "dmrg_env_update")
 
     def svd_split(self, field, M, m, n, cutoff, maxdim, mindim, absorb):
@@ -162,7 +170,7 @@ class Engine:
         npl = 2 if field == 1 else 1
         G = np.zeros((npl, wl, wr, d, d))
         check(self.lib.dmrg_mpo_grad(self.ctx, field, prec, _ptr(L), _ptr(bra), _ptr(ket), _ptr(R), Dl, Dr, Dlb,
-                                     Drb, d, wl, wr, _hptr(G)), "dmrg_mpo_grad")
+                                     Drb, d, wl, wr, int(L.shape[-1]), int(R.shape[-1]), _hptr(G)), "dmrg_mpo_grad")
         return G
 
     def gemm(self, prec, field, A, B, C, M, N, K, transA=False, transB=False, conjA=False, conjB=False,

@@ -43,6 +43,8 @@ typedef struct dmrg_heff_desc {
   const double* W1;    /* host, nsite 2: [wl][wm][d1][d1]; nsite 1: [wl][wr][d1][d1] */
   const double* W2;    /* host, [wm][wr][d2][d2] (nsite 2 only) */
   int w_complex;       /* W1/W2 carry an imaginary plane after the real one */
+  int ldL, ldR;        /* row pitch (elements) of the last index of L and R; 0 = Dl / Dr.  A pitch
+                          that is a multiple of 4 lets every H_eff GEMM use 16-byte loads */
 } dmrg_heff_desc;
 
 /* One environment extension <bra|W|ket> by one site. */
@@ -59,6 +61,7 @@ typedef struct dmrg_env_desc {
   int w_complex;
   const void* env_in;  /* device: dir 0 [wl][Dlb][Dl]; dir 1 [wr][Dr][Drb] */
   void* env_out;       /* device: dir 0 [wr][Drb][Dr]; dir 1 [wl][Dl][Dlb] */
+  int ld_in, ld_out;   /* row pitch of the last index of env_in / env_out; 0 = dense */
 } dmrg_env_desc;
 
 /* Library version (major*100 + minor). */
@@ -123,11 +126,13 @@ int dmrg_svd_split(dmrg_ctx* ctx, int field, const double* M, int m, int n, doub
 /* Gradient of <bra|H|ket> with respect to one MPO site tensor W[wl][wr][s'][s]:
  * G = sum L[wl][a'][a] conj(bra[a'][s'][b']) ket[a][s][b] R[wr][b][b'], written
  * to host memory as G[np][wl][wr][d][d] (FP64).  L [wl][Dlb][Dl] and
- * R [wr][Dr][Drb] are environments in the contraction precision.  Has no
+ * R [wr][Dr][Drb] are environments in the contraction precision, with row
+ * pitches ldL >= Dl and ldR >= Drb of their last index (0 = dense).  Has no
  * counterpart in the reference (which has no gradients); with a normalised
  * ground state it gives dE/dW (Hellmann-Feynman) for every coupling. */
 int dmrg_mpo_grad(dmrg_ctx* ctx, int field, int prec, const void* L, const double* bra, const double* ket,
-                  const void* R, int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, double* G_host);
+                  const void* R, int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, int ldL, int ldR,
+                  double* G_host);
 
 /* General batched GEMM in a given precision (used by the gradient
  * contractions and by the tests): C = alpha op(A) op(B) + beta C with the

@@ -30,11 +30,31 @@ def test_binding_matches_header():
 
 
 def test_struct_layouts():
-    # 10 ints then pointers: the C layout puts the first pointer at offset 40
-    assert _lib.HeffDesc.L.offset == 40
-    assert ctypes.sizeof(_lib.HeffDesc) == 40 + 4 * 8 + 8
-    assert _lib.EnvDesc.ket.offset == 40
-    assert ctypes.sizeof(_lib.EnvDesc) == 40 + 3 * 8 + 8 + 2 * 8
+    """ctypes descriptors against the C layout of include/dmrg_hip.h, as gcc
+    lays it out (offset of every field and the struct size)."""
+    import os
+    import subprocess
+    import tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    checks = []
+    for cname, py in (("dmrg_heff_desc", _lib.HeffDesc), ("dmrg_env_desc", _lib.EnvDesc)):
+        checks.append('printf("%s sizeof %zu\\n", sizeof({0}));'.replace("%s", cname).format(cname))
+        for f, _ in py._fields_:
+            checks.append('printf("{0} {1} %zu\\n", offsetof({0}, {1}));'.format(cname, f))
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"dmrg_hip.h\"\nint main(void) {\n" + \
+        "\n".join(checks) + "\nreturn 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "layout.c"), "w") as f:
+            f.write(src)
+        exe = os.path.join(d, "layout")
+        subprocess.run(["gcc", "-I", os.path.join(root, "include"), os.path.join(d, "layout.c"), "-o", exe],
+                       check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in (("dmrg_heff_desc", _lib.HeffDesc), ("dmrg_env_desc", _lib.EnvDesc)):
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
 
 
 def test_version_and_error_are_callable_without_gpu():

@@ -26,7 +26,10 @@ def planar(a):
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("cplx", [False, True])
-def test_mpo_grad_kernel(engine, prec, cplx):
+@pytest.mark.parametrize("pitch", [0, 3])
+def test_mpo_grad_kernel(engine, prec, cplx, pitch):
+    """G = d<bra|H|ket>/dW against einsum; with pitch > 0 the environments'
+    last index is padded with NaN columns that must not be read."""
     import torch
     rng = np.random.default_rng(3 + prec + 2 * cplx)
     Dl, Dr, Dlb, Drb, d, wl, wr = 9, 13, 7, 11, 3, 4, 5
@@ -37,7 +40,15 @@ def test_mpo_grad_kernel(engine, prec, cplx):
     ref = np.einsum("lpa,pqB,asb,rbB->lrqs", L, bra.conj(), ket, R)
     et = engine.env_dtype(prec)
     dv = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(planar(a))).to(dt).to(engine.device)
-    G = engine.mpo_grad(int(cplx), prec, dv(L, et), dv(bra), dv(ket), dv(R, et), wl, wr)
+
+    def pad(t):
+        if not pitch:
+            return t
+        out = torch.full(t.shape[:-1] + (t.shape[-1] + pitch,), float("nan"), dtype=t.dtype, device=t.device)
+        out[..., :t.shape[-1]] = t
+        return out
+
+    G = engine.mpo_grad(int(cplx), prec, pad(dv(L, et)), dv(bra), dv(ket), pad(dv(R, et)), wl, wr)
     Gc = G[0] + 1j * G[1] if cplx else G[0]
     err = np.linalg.norm(Gc - ref) / np.linalg.norm(ref)
     assert err < (1e-12 if prec == 0 else 1e-5), err

@@ -166,6 +166,54 @@ def test_heff2_real_mpo(engine, prec, cplx, dims):
     assert err < TOL[prec] * 10, err
 
 
+def pitched(engine, a, dtype, extra=3):
+    """Device copy of a planar array whose last index has `extra` padding
+    columns filled with NaN: a kernel that reads the padding poisons its
+    output."""
+    import torch
+    t = dev(engine, a, dtype)
+    out = torch.full(t.shape[:-1] + (t.shape[-1] + extra,), float("nan"), dtype=t.dtype, device=t.device)
+    out[..., :t.shape[-1]] = t
+    return out
+
+
+def unpitch(t, n):
+    return t[..., :n].contiguous()
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("nsite", [1, 2])
+def test_heff_pitched_environments(engine, prec, cplx, nsite):
+    """Environments whose last index has a row pitch (padding = NaN): the
+    H_eff apply reads only the logical columns; d = 3 makes theta's rows
+    (9 Dr) need a pitched copy as well."""
+    import torch
+    rng = np.random.default_rng(31 + prec + 2 * cplx + 5 * nsite)
+    Dl, Dr, d, w = 21, 35, 3, 4
+    L = rnd(rng, (Dl, w, Dl), cplx)
+    R = rnd(rng, (Dr, w, Dr), cplx)
+    W1 = rnd(rng, (w, d, d, w), False)
+    W2 = rnd(rng, (w, d, d, w), False)
+    et = engine.env_dtype(prec)
+    if nsite == 2:
+        th = rnd(rng, (Dl, d, d, Dr), cplx)
+        ref = odmrg.heff2_matvec(L, W1, W2, R, th.shape)(th.reshape(-1).astype(complex)).reshape(th.shape)
+        desc = engine.heff_desc(pitched(engine, L_eng(L), et), pitched(engine, R_eng(R), et, 1),
+                                W_eng(W1), W_eng(W2), Dl, Dr, d, d, int(cplx), prec)
+    else:
+        th = rnd(rng, (Dl, d, Dr), cplx)
+        ref = odmrg.heff1_matvec(L, W1, R, th.shape)(th.reshape(-1).astype(complex)).reshape(th.shape)
+        desc = engine.heff_desc(pitched(engine, L_eng(L), et), pitched(engine, R_eng(R), et, 1),
+                                W_eng(W1), None, Dl, Dr, d, 0, int(cplx), prec)
+    x = dev(engine, th)
+    y = engine.zeros(x.shape)
+    engine.heff_apply(desc, x, y)
+    torch.cuda.synchronize()
+    err = relerr(host(y), ref if cplx else ref.real)
+    assert err < TOL[prec] * 10, err
+
+
 @pytest.mark.parametrize("prec", [0, 1, 2])
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("model", ["hubbard", "heisenberg"])
@@ -225,7 +273,11 @@ def test_heff1(engine, prec, cplx):
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("direction", ["left", "right"])
-def test_env(engine, prec, cplx, direction):
+@pytest.mark.parametrize("pitch", [0, 3])
+def test_env(engine, prec, cplx, direction, pitch):
+    """Environment update against the oracle; with pitch > 0 the input
+    environment's padding columns are NaN and the output is written into a
+    pitched buffer (padding untouched)."""
     import torch
     rng = np.random.default_rng(31 + prec)
     Dl, Dr, Dlb, Drb, d, wl, wr = 19, 27, 13, 21, 3, 4, 5
@@ -236,18 +288,20 @@ def test_env(engine, prec, cplx, direction):
     if direction == "left":
         Lin = rnd(rng, (Dlb, wl, Dl), cplx)
         ref = omps.extend_left(Lin, bra, W, ket)               # [Drb, wr, Dr]
-        ein = dev(engine, L_eng(Lin), et)
-        eout = engine.zeros((ein.shape[0], wr, Drb, Dr), dtype=et)
-        ref_e = L_eng(ref)
+        ein = pitched(engine, L_eng(Lin), et, pitch) if pitch else dev(engine, L_eng(Lin), et)
+        eout = engine.zeros((ein.shape[0], wr, Drb, Dr + pitch), dtype=et)
+        ref_e, n_out = L_eng(ref), Dr
     else:
         Rin = rnd(rng, (Drb, wr, Dr), cplx)
         ref = omps.extend_right(Rin, bra, W, ket)              # [Dlb, wl, Dl]
-        ein = dev(engine, R_eng(Rin), et)
-        eout = engine.zeros((ein.shape[0], wl, Dl, Dlb), dtype=et)
-        ref_e = R_eng(ref)
+        ein = pitched(engine, R_eng(Rin), et, pitch) if pitch else dev(engine, R_eng(Rin), et)
+        eout = engine.zeros((ein.shape[0], wl, Dl, Dlb + pitch), dtype=et)
+        ref_e, n_out = R_eng(ref), Dlb
     engine.env_update(direction, int(cplx), prec, dev(engine, ket), dev(engine, bra), W_eng(W), ein, eout)
     torch.cuda.synchronize()
-    assert relerr(host(eout), ref_e if cplx else ref_e.real) < TOL[prec] * 10
+    assert relerr(host(unpitch(eout, n_out)), ref_e if cplx else ref_e.real) < TOL[prec] * 10
+    if pitch:
+        assert float(eout[..., n_out:].abs().max()) == 0.0
 
 
 @pytest.mark.parametrize("cplx", [False, True])
