@@ -210,6 +210,72 @@ def workload_tdvp(args):
     print(json.dumps(out))
 
 
+SMALL_CPU = r"""
+import json, sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle import dmrg as od, mpo as omp, mps as om, ed
+n, D, ns = int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4])
+np.random.seed(0)
+psi = om.random_mps([2] * n, D)
+Ws = omp.to_mpo([2] * n, ed.heisenberg_terms(n))
+t0 = time.perf_counter()
+es = []
+od.dmrg(psi, Ws, [D] * ns, 1e-12, 2, callback=lambda k, e, p: es.append(e))
+print(json.dumps({"t": time.perf_counter() - t0, "energies": es}))
+"""
+
+
+def workload_small(args):
+    """The reference's CPU-runnable configuration (configuration 1): open
+    S=1/2 Heisenberg chain N=32, D=64, 6 sweeps, FP64, random initial MPS
+    from np.random.seed(0) drawn as the reference draws it.  The GPU session
+    and the oracle (the reference algorithm in NumPy, one full run on the
+    host, 1 BLAS thread and all cores) run the same sweeps from the same
+    state; their per-sweep energies must agree to 1e-10 per site."""
+    import numpy as np
+    import torch
+    from dmrgpy_amd.chain import Chain
+    from dmrgpy_amd.multioperator import heisenberg_terms
+    n = args.n or 32
+    D = 64 if args.D == 1024 else args.D
+    ns = 6
+    ch = Chain([2] * n)
+    ch.set_sweep_params(D, ns, 1e-12, 0.0)
+    ch.set_hamiltonian(heisenberg_terms(n))
+    np.random.seed(0)
+    ch.wf0 = ch._default_mps()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ch.gs_energy()
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    es = list(ch.sweep_energies)
+    cpu = {}
+    for threads in (1, args.cpu_threads):
+        env = dict(os.environ)
+        for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS"):
+            env[k] = str(threads)
+        p = subprocess.run([sys.executable, "-c", SMALL_CPU, ROOT, str(n), str(D), str(ns)], env=env,
+                           capture_output=True, text=True, timeout=1800)
+        if p.returncode != 0:
+            log("oracle run failed:", p.stderr[-2000:])
+            continue
+        cpu[threads] = json.loads(p.stdout.strip().splitlines()[-1])
+    ref = cpu.get(args.cpu_threads) or cpu.get(1)
+    dE = max(abs(a - b) for a, b in zip(es, ref["energies"])) / n if ref else None
+    out = {"metric": "DMRG time of the reference's CPU configuration (N=32, D=64, 6 sweeps, FP64)",
+           "value": ns / t_gpu, "unit": "sweeps/s", "n_gpus": 1, "steps": ns, "warmup": 0,
+           "ms_per_step": t_gpu / ns * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp64", "data": "synthetic: seeded random initial MPS (np.random.seed(0))",
+           "config": {"workload": "two-site DMRG, open S=1/2 Heisenberg N={}, D={}, {} sweeps".format(n, D, ns),
+                      "N": n, "D": D},
+           "energies": es, "max_dE_per_site_vs_oracle": dE,
+           "cpu_baseline": [{"value": ns / r["t"], "unit": "sweeps/s", "cores": t, "kind": "port",
+                             "sample": "the full 6-sweep run of the oracle"} for t, r in sorted(cpu.items())]}
+    print(json.dumps(out))
+
+
 def workload_gradients(args):
     """Coupling gradients (configuration 4, sizes from the flags): random-bond
     XXZ chain, J_i, Delta_i ~ U[0.5, 1.5] from default_rng(seed 4), D cap,
@@ -287,7 +353,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--workload", default="dmrg", choices=["dmrg", "tdvp", "gradients"],
+    ap.add_argument("--workload", default="dmrg", choices=["dmrg", "small", "tdvp", "gradients"],
                     help="dmrg: the headline sweep benchmark; tdvp / gradients: the TDVP-quench and "
                          "coupling-gradient configurations (sizes from --n/--D/--steps)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -305,6 +371,9 @@ def main():
         return
     if args.workload == "gradients":
         workload_gradients(args)
+        return
+    if args.workload == "small":
+        workload_small(args)
         return
 
     import numpy as np

@@ -24,7 +24,8 @@ class HeffDesc(ctypes.Structure):
                 ("wl", c_int), ("wm", c_int), ("wr", c_int),
                 ("L", c_void_p), ("R", c_void_p),
                 ("W1", c_void_p), ("W2", c_void_p), ("w_complex", c_int),
-                ("ldL", c_int), ("ldR", c_int)]
+                ("ldL", c_int), ("ldR", c_int),
+                ("nproj", c_int), ("proj", c_void_p), ("proj_weight", c_double)]
 
 
 class EnvDesc(ctypes.Structure):

@@ -147,6 +147,91 @@ def dmrg_sweep(eng, psi, mpo, maxdim, cutoff, niter, prec=0, stats=None, tol=LAN
     return energy
 
 
+# ---------------------------------------------------------------------------
+# excited states: overlap-penalty DMRG (src/dmrgpy/pyitensor/dmrg.py:520-681)
+# ---------------------------------------------------------------------------
+def _ones_mpo_site(d):
+    return np.eye(d)[None, None, None]
+
+
+def overlap_left(eng, L, wk, psi, i):
+    """<wk|psi> left environment one site further ([1][D_wk][D_psi])."""
+    return env_left(eng, L, psi.A(i), _ones_mpo_site(psi.A(i).shape[2]), psi.field, 0, bra=wk.A(i))
+
+
+def overlap_right(eng, R, wk, psi, i):
+    return env_right(eng, R, psi.A(i), _ones_mpo_site(psi.A(i).shape[2]), psi.field, 0, bra=wk.A(i))
+
+
+def bond_projection(eng, Lov, wk, Rov, i, dims, field):
+    """wk projected onto psi's two-site basis at bond i (dmrg.py:556-561):
+    p[a][s1 s2][b] = sum_{a' b'} conj(Lov[a'][a]) theta_wk[a'][s1 s2][b'] conj(Rov[b][b'])."""
+    Dl, d1, d2, Dr = dims
+    th, (Dla, _, _, Drb) = theta2(eng, wk, i)
+    npl = 2 if field else 1
+    ldL, ldR = Lov.shape[-1], Rov.shape[-1]
+    T = eng.empty((npl, Dl * d1 * d2 * Drb))
+    eng.gemm(0, field, Lov, th, T, Dl, d1 * d2 * Drb, Dla, transA=True, conjA=bool(field), lda=ldL,
+             planeA=Dla * ldL, planeB=Dla * d1 * d2 * Drb, planeC=Dl * d1 * d2 * Drb)
+    P = eng.empty((npl, Dl * d1 * d2 * Dr))
+    eng.gemm(0, field, T, Rov, P, Dl * d1 * d2, Dr, Drb, transB=True, conjB=bool(field), ldb=ldR,
+             planeA=Dl * d1 * d2 * Drb, planeB=Dr * ldR, planeC=Dl * d1 * d2 * Dr)
+    return P
+
+
+def dmrg_sweep_excited(eng, psi, mpo, penalty, weight, maxdim, cutoff, niter, prec=0, stats=None,
+                       tol=LANCZOS_TOL, res_tol=0.0):
+    """One penalised sweep (dmrg_excited, dmrg.py:620-681): the local
+    operator is H_eff + weight * sum_k |p_k><p_k| with p_k the projection of
+    the k-th penalty state onto the current two-site basis; overlap
+    environments <wk|psi> are rebuilt from the right at the start of the
+    sweep and extended with every update, like the H environments."""
+    n = psi.length()
+    field = psi.field
+    energy = None
+    R = right_envs(eng, psi, mpo, prec)
+    one = lambda: ones_env(eng, field, 0)
+    Rov = []
+    for wk in penalty:
+        env = {n + 1: one()}
+        for i in range(n, 1, -1):
+            env[i] = overlap_right(eng, env[i + 1], wk, psi, i)
+        Rov.append(env)
+    L = {0: ones_env(eng, field, prec)}
+    Lov = [{0: one()} for _ in penalty]
+
+    def solve(i):
+        th, dims = theta2(eng, psi, i)
+        desc = heff2(eng, L[i - 1], R[i + 2], mpo, i, dims, field, prec)
+        if penalty:
+            import torch
+            P = torch.stack([bond_projection(eng, Lov[k][i - 1], wk, Rov[k][i + 2], i, dims, field)
+                             for k, wk in enumerate(penalty)])
+            eng.set_projectors(desc, P, weight)
+        e, it = eng.lanczos(desc, th, max(niter, LANCZOS_CAP), tol, res_tol)
+        if stats is not None:
+            stats.lanczos_iters.append(it)
+        return e, th, dims
+
+    for i in range(1, n):
+        energy, th, dims = solve(i)
+        update(eng, psi, i, th, dims, cutoff, maxdim, "right", stats)
+        L[i] = env_left(eng, L[i - 1], psi.A(i), mpo.W[i - 1], field, prec)
+        for k, wk in enumerate(penalty):
+            Lov[k][i] = overlap_left(eng, Lov[k][i - 1], wk, psi, i)
+        R.pop(i + 2, None)
+    R = {n + 1: ones_env(eng, field, prec)}
+    Rov = [{n + 1: one()} for _ in penalty]
+    for i in range(n - 1, 0, -1):
+        energy, th, dims = solve(i)
+        update(eng, psi, i, th, dims, cutoff, maxdim, "left", stats)
+        R[i + 1] = env_right(eng, R[i + 2], psi.A(i + 1), mpo.W[i], field, prec)
+        for k, wk in enumerate(penalty):
+            Rov[k][i + 1] = overlap_right(eng, Rov[k][i + 2], wk, psi, i + 1)
+        L.pop(i, None)
+    return energy
+
+
 def expectation(eng, bra, mpo, ket, prec=0):
     """<bra|M|ket> by a full left-to-right environment sweep (mpsalgebra.py:99-114)."""
     field = max(bra.field, ket.field, 1 if mpo.complex else 0)

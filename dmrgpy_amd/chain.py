@@ -87,6 +87,7 @@ class Chain:
         self.H_terms = list(terms)
         self.have_H = True
         self._wf0_energy = None
+        self._bandwidth_max = None
 
     def build_operator(self, terms):
         return build_mpo(self.site_types, terms)
@@ -195,6 +196,66 @@ class Chain:
 
     def _apply_mpo(self, K, x):
         return alg.apply_mpo(self.eng, K, x, self.cutoff, self.maxm)
+
+    # -- excited states (chain.py:253-275, 1201-1231) ---------------------
+    def _maximum_energy(self):
+        """Reduced-effort DMRG on -H from a fresh random MPS (<= 5 sweeps,
+        maxdim <= 20): a spectral bound for the penalty weight."""
+        if getattr(self, "_bandwidth_max", None) is None:
+            psi = self._default_mps()
+            neg = build_mpo(self.site_types, [(-complex(c), f) for c, f in self.H_terms])
+            self._match_field(psi, neg)
+            e = None
+            for _ in range(min(self.nsweeps, 5)):
+                e = alg.dmrg_sweep(self.eng, psi, neg, min(self.maxm, 20), self.cutoff, self.niter, 0)
+            self._bandwidth_max = -e
+        return self._bandwidth_max
+
+    def _bandwidth(self):
+        return self._maximum_energy() - self.gs_energy(skip_dmrg=True)
+
+    def _energy_fluctuation(self, psi, H):
+        """<H^2> - <H>^2 of the normalised state, with H|psi> by zip-up."""
+        p = psi.copy()
+        alg.normalize(self.eng, p)
+        hp = self._apply_mpo(H, p)
+        de = alg.overlap(self.eng, p, hp).real
+        return alg.overlap(self.eng, hp, hp).real - de * de
+
+    def excited_states(self, n, scale_lagrange=1.0, do_gram_schmidt=False):
+        """The n lowest eigenstates by overlap-penalty DMRG (chain.py:253-275):
+        state k minimises <psi|H + w sum_{j<k} |psi_j><psi_j| |psi> from a
+        fresh random MPS, w = bandwidth * scale_lagrange.  Returns
+        (energies <psi|H|psi>, fluctuations <H^2> - <H>^2, wave functions)."""
+        if not self.have_H:
+            raise RuntimeError("Chain.excited_states called before set_hamiltonian")
+        if do_gram_schmidt:
+            raise NotImplementedError("do_gram_schmidt needs MPS sums, which this backend does not provide")
+        if self.wf0 is None:
+            self.gs_energy()
+        psi0 = self.wf0.copy()
+        alg.normalize(self.eng, psi0)
+        wfs = [psi0]
+        weight = self._bandwidth() * scale_lagrange
+        for _ in range(1, int(n)):
+            psi1 = self._default_mps()
+            self._match_field(psi1, self.H)
+            if psi1.field != psi0.field:
+                wfs = [w.as_complex() if not w.field else w for w in wfs]
+                psi1 = psi1.as_complex() if not psi1.field else psi1
+            for k in range(self.nsweeps):
+                maxdim = self.maxdim_schedule[min(k, len(self.maxdim_schedule) - 1)] if self.maxdim_schedule \
+                    else self.maxm
+                prec = self._prec(k)
+                alg.dmrg_sweep_excited(self.eng, psi1, self.H, wfs, weight, maxdim, self.cutoff, self.niter,
+                                       prec, self.stats, res_tol=RES_TOL.get(prec, 0.0))
+            alg.normalize(self.eng, psi1)
+            wfs.append(psi1)
+        energies, fluct = [], []
+        for wf in wfs:
+            fluct.append(self._energy_fluctuation(wf, self.H))
+            energies.append((alg.expectation(self.eng, wf, self.H, wf) / alg.overlap(self.eng, wf, wf)).real)
+        return energies, fluct, wfs
 
     def _dmrg(self, psi, H, callback=None):
         energy = None

@@ -86,6 +86,10 @@ __global__ void convert_back_k(const T* __restrict__ src, double* __restrict__ d
   for (; i < n; i += stride) dst[i] = (double)src[i];
 }
 
+__global__ void scale_vec_k(double* __restrict__ v, int n, double s) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] *= s;
+}
+
 static unsigned grid1d(long long n) {
   long long b = (n + 255) / 256;
   if (b > 65536) b = 65536;
@@ -491,6 +495,8 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   DMRG_REQUIRE(!(d.w_complex && d.field == REAL), "heff: complex MPO requires complex field");
   DMRG_REQUIRE((d.ldL == 0 || d.ldL >= d.Dl) && (d.ldR == 0 || d.ldR >= d.Dr),
                "heff: environment row pitch smaller than the dimension");
+  DMRG_REQUIRE(d.nproj >= 0 && d.nproj <= 64 && (d.nproj == 0 || d.proj != nullptr),
+               "heff: bad penalty projectors");
   const int Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr;
   const bool wc = d.w_complex != 0;
   if (d.nsite == 2) {
@@ -615,6 +621,15 @@ void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
     gemm_prec(c, d.prec, g, T2, d.R, yout);
   }
   if (d.prec != P64) from_prec(c, d.prec, yout, y, n * np);
+  if (d.nproj > 0) {
+    // y += w sum_k p_k <p_k|x>  (FP64; coefficients stay on the device)
+    double* coef = (double*)ws_get(c, WS_PROJ, sizeof(double) * 2 * (size_t)d.nproj);
+    const long long ldp = n * np;
+    vec_dot_block(c, d.field, d.proj, ldp, d.nproj, x, n, coef);
+    scale_vec_k<<<1, 128, 0, c->stream>>>(coef, (d.field == CPLX ? 2 : 1) * d.nproj, -d.proj_weight);
+    DMRG_CHECK_HIP(hipGetLastError());
+    vec_sub_block(c, d.field, d.proj, ldp, d.nproj, coef, y, n);
+  }
   c->matvecs++;
 }
 

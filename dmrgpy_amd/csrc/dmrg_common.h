@@ -72,7 +72,7 @@ struct DevBuf {
 enum Slot {
   WS_T1 = 0, WS_T2, WS_XIN, WS_YOUT, WS_WSMALL, WS_ENV1, WS_ENV2,
   WS_KRYLOV_Q, WS_KRYLOV_W, WS_KRYLOV_RED, WS_KRYLOV_COEF,
-  WS_SVD_GRAM, WS_SVD_EV, WS_SVD_TMP, WS_SVD_TMP2, WS_SVD_INFO, WS_SVD_CPLX, WS_SVD_FIX, WS_SVD_RED,
+  WS_SVD_GRAM, WS_SVD_EV, WS_SVD_TMP, WS_SVD_TMP2, WS_SVD_INFO, WS_SVD_CPLX, WS_SVD_FIX, WS_SVD_RED, WS_PROJ,
   WS_BF16_A, WS_BF16_B, WS_REFINE,
   WS_NSLOTS
 };

@@ -621,8 +621,14 @@ static void gemm_any(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
   long long tiles128 = (long long)((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch;
   if (d.field == REAL) {
     const bool vec = vec_ok(d, A, B);
-    if (tiles128 >= 256) launch_fast<T, 4, 4>(c, d, A, B, C, vec);
-    else launch_fast<T, 2, 2>(c, d, A, B, C, vec);
+    // FP64 uses 128x64 tiles: a 128x128 FP64 tile needs ~380 registers per
+    // lane (one wave per SIMD); 128x64 keeps two
+    if (tiles128 >= 256) {
+      if (sizeof(T) == 8) launch_fast<T, 4, 2>(c, d, A, B, C, vec);
+      else launch_fast<T, 4, 4>(c, d, A, B, C, vec);
+    } else {
+      launch_fast<T, 2, 2>(c, d, A, B, C, vec);
+    }
     return;
   }
   long long tiles64 = (long long)((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch;

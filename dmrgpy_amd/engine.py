@@ -100,8 +100,23 @@ class Engine:
         d.L = _ptr(L)
         d.R = _ptr(R)
         d.ldL, d.ldR = int(L.shape[-1]), int(R.shape[-1])   # row pitch of the last index
+        d.nproj, d.proj, d.proj_weight = 0, None, 0.0
         d._hold = (W1, W2, L, R)
         return d
+
+    @staticmethod
+    def set_projectors(desc, P, weight):
+        """Penalty projectors for an H_eff descriptor: P float64 device
+        [nproj, np, n] (the layout of the vectors the operator acts on); the
+        apply then adds weight * sum_k p_k <p_k|x>."""
+        if P is None or P.shape[0] == 0:
+            desc.nproj, desc.proj, desc.proj_weight = 0, None, 0.0
+            return desc
+        desc.nproj = int(P.shape[0])
+        desc.proj = _ptr(P)
+        desc.proj_weight = float(weight)
+        desc._hold = desc._hold + (P,)
+        return desc
 
     # -- entry points -----------------------------------------------------
     def heff_apply(self, desc, x, y):

@@ -45,6 +45,9 @@ typedef struct dmrg_heff_desc {
   int w_complex;       /* W1/W2 carry an imaginary plane after the real one */
   int ldL, ldR;        /* row pitch (elements) of the last index of L and R; 0 = Dl / Dr.  A pitch
                           that is a multiple of 4 lets every H_eff GEMM use 16-byte loads */
+  int nproj;           /* penalty projectors (excited states, src/dmrgpy/pyitensor/dmrg.py:562-599): */
+  const double* proj;  /* device FP64 [nproj][np][n] vectors p_k in the layout of x; the apply adds */
+  double proj_weight;  /* proj_weight * sum_k p_k <p_k|x>; nproj = 0 for the plain H_eff */
 } dmrg_heff_desc;
 
 /* One environment extension <bra|W|ket> by one site. */

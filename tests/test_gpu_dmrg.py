@@ -230,3 +230,37 @@ def test_reduced_precision_sweeps_keep_isometries():
     e_ray = (alg.expectation(ch.eng, psi, ch.H, psi) / alg.overlap(ch.eng, psi, psi)).real
     assert e64 > e_ed - 1e-9 and e_ray > e_ed - 1e-9
     assert abs(e_ray - e_ed) / n < 1e-8
+
+
+def test_excited_states_vs_ed():
+    """Overlap-penalty excited states (chain.py:253-275, dmrg.py:520-681)
+    against exact diagonalisation: random-bond XXZ chain with random
+    longitudinal fields (non-degenerate spectrum), N=8 at full bond
+    dimension; energies, vanishing energy fluctuations, mutual orthogonality."""
+    from dmrgpy_amd import algorithms as alg
+    from dmrgpy_amd.chain import Chain
+    from dmrgpy_amd.gradients import xxz_terms
+    from oracle import ed
+    rng = np.random.default_rng(23)
+    n = 8
+    terms = xxz_terms(rng.uniform(0.5, 1.5, n - 1), rng.uniform(0.5, 1.5, n - 1))
+    terms += [(complex(h), [("Sz", i + 1)]) for i, h in enumerate(rng.uniform(-0.4, 0.4, n))]
+    w = np.linalg.eigvalsh(ed.sparse_hamiltonian([2] * n, terms).toarray())
+    ch = Chain([2] * n, field="real")
+    ch.set_sweep_params(16, 10, 1e-12, 0.0)
+    ch.set_hamiltonian(terms)
+    np.random.seed(1)
+    energies, fluct, wfs = ch.excited_states(3, scale_lagrange=2.0)
+    assert np.abs(np.array(energies) - w[:3]).max() < 1e-8, (energies, w[:3])
+    # the session's fluctuation uses H|psi> by zip-up truncated at maxm, as
+    # the reference does (chain.py:1225-1231); the exact variance is dense
+    assert max(fluct) < 1e-6, fluct
+    Hd = ed.sparse_hamiltonian([2] * n, terms).toarray()
+    for wf in wfs:
+        v = wf.to_dense().reshape(-1)
+        v = v / np.linalg.norm(v)
+        hv = Hd @ v
+        assert abs(np.vdot(hv, hv) - np.vdot(v, hv) ** 2) < 1e-9
+    for a in range(3):
+        for b in range(a):
+            assert abs(alg.overlap(ch.eng, wfs[a], wfs[b])) < 1e-6
