@@ -197,6 +197,47 @@ class Chain:
     def _apply_mpo(self, K, x):
         return alg.apply_mpo(self.eng, K, x, self.cutoff, self.maxm)
 
+    # -- generalized eigenproblem (chain.py:230-251, dmrg.py:425-517) -------
+    def gs_energy_generalized(self, terms_a, lam0=None):
+        """Smallest lambda of H|psi> = lambda A|psi> (A Hermitian positive
+        definite): each sweep is an ordinary DMRG sweep on H - lambda A
+        (built as one automaton MPO from both term lists), after which lambda
+        becomes <psi|H|psi>/<psi|A|psi>; lambda starts from lam0 or, when
+        lam0 is None or NaN, from the current state's quotient.  Mutates
+        wf0 and invalidates the plain-energy cache, like the reference."""
+        if not self.have_H:
+            raise RuntimeError("Chain.gs_energy_generalized called before set_hamiltonian")
+        A = build_mpo(self.site_types, terms_a)
+        if self.wf0 is None:
+            self.wf0 = self._default_mps()
+        psi = self.wf0
+        self._match_field(psi, self.H)
+        self._match_field(psi, A)
+        ratio = lambda: (alg.expectation(self.eng, psi, self.H, psi).real,
+                         alg.expectation(self.eng, psi, A, psi).real)
+        unset = lam0 is None or np.isnan(complex(lam0).real)
+        if unset:
+            h0, a0 = ratio()
+            lam = h0 / a0 if abs(a0) > 1e-14 else 0.0
+        else:
+            lam = complex(lam0).real
+        for k in range(self.nsweeps):
+            maxdim = self.maxdim_schedule[min(k, len(self.maxdim_schedule) - 1)] if self.maxdim_schedule \
+                else self.maxm
+            Hl = build_mpo(self.site_types, list(self.H_terms) + [(-lam * complex(c), f) for c, f in terms_a])
+            self._match_field(psi, Hl)
+            prec = self._prec(k)
+            alg.dmrg_sweep(self.eng, psi, Hl, maxdim, self.cutoff, self.niter, prec, self.stats,
+                           res_tol=RES_TOL.get(prec, 0.0))
+            h, a = ratio()
+            if not abs(a) > 1e-14:
+                raise RuntimeError("gs_energy_generalized: <psi|A|psi> collapsed to ~0 (or NaN)")
+            lam = h / a
+            if self.verbose:
+                print("sweep {}: lambda = {}".format(k, lam))
+        self._wf0_energy = None
+        return lam
+
     # -- excited states (chain.py:253-275, 1201-1231) ---------------------
     def _maximum_energy(self):
         """Reduced-effort DMRG on -H from a fresh random MPS (<= 5 sweeps,

@@ -135,6 +135,63 @@ __global__ void convert_pitched_k(const double* __restrict__ src, T* __restrict_
   }
 }
 
+// hi/lo bf16 split of FP32/FP64 rows: dst[r][c] (pitch ldd) for c < cols
+template <typename S>
+__global__ void split_rows_k(const S* __restrict__ src, long long rows, int cols, int lds, __bf16* __restrict__ hi,
+                             __bf16* __restrict__ lo, int ldd) {
+  const long long tot = rows * cols;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < tot; i += stride) {
+    const long long r = i / cols;
+    const int col = (int)(i - r * cols);
+    const float x = (float)src[r * lds + col];
+    const __bf16 h = (__bf16)x;
+    hi[r * ldd + col] = h;
+    lo[r * ldd + col] = (__bf16)(x - (float)h);
+  }
+}
+
+// hi/lo bf16 split with transposition: dst[c][r] (pitch ldd) = src[r][c]
+// (pitch lds); 32x32 tiles through LDS so reads and writes are coalesced
+template <typename S>
+__global__ __launch_bounds__(256) void split_transpose_k(const S* __restrict__ src, int rows, int cols, int lds,
+                                                         __bf16* __restrict__ hi, __bf16* __restrict__ lo, int ldd) {
+  __shared__ float tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = r0 + ty + 8 * j, col = c0 + tx;
+    tile[ty + 8 * j][tx] = (r < rows && col < cols) ? (float)src[(long long)r * lds + col] : 0.f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int col = c0 + ty + 8 * j, r = r0 + tx;
+    if (col < cols && r < rows) {
+      const float x = tile[tx][ty + 8 * j];
+      const __bf16 h = (__bf16)x;
+      hi[(long long)col * ldd + r] = h;
+      lo[(long long)col * ldd + r] = (__bf16)(x - (float)h);
+    }
+  }
+}
+
+template <typename S>
+void split_rows(Ctx* c, const S* src, long long rows, int cols, int lds, __bf16* hi, __bf16* lo, int ldd) {
+  split_rows_k<S><<<grid1d(rows * cols), 256, 0, c->stream>>>(src, rows, cols, lds, hi, lo, ldd);
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
+template <typename S>
+void split_transpose(Ctx* c, const S* src, int rows, int cols, int lds, __bf16* hi, __bf16* lo, int ldd) {
+  DMRG_REQUIRE((rows + 31) / 32 <= 65535, "split_transpose: too many rows");
+  dim3 g((cols + 31) / 32, (rows + 31) / 32);
+  split_transpose_k<S><<<g, 256, 0, c->stream>>>(src, rows, cols, lds, hi, lo, ldd);
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
 void to_prec_pitched(Ctx* c, int prec, const double* src, void* dst, long long rows, int cols, int ld, int planes) {
   if (ld == cols) {
     to_prec(c, prec, src, dst, rows * cols * planes);
@@ -242,13 +299,14 @@ SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& w
 // ---------------------------------------------------------------------------
 constexpr int SP_BN = 64, SP_LDI = SP_BN + 1, SP_CHUNKS = 8, SP_KMAX = 128, SP_PRE = SP_KMAX * SP_BN / 256;
 
-template <typename T>
+template <typename T, bool SPLIT>
 __global__ __launch_bounds__(256) void smallop_sparse_k(const T* __restrict__ in, T* __restrict__ out,
                                                         const int* __restrict__ rowptr,
                                                         const int* __restrict__ colidx, const T* __restrict__ val,
                                                         const int* __restrict__ grp, const int* __restrict__ ioff,
                                                         const int* __restrict__ ooff, int inner, long long sx_in,
-                                                        long long sx_out, int Kin, long long pin, long long pout) {
+                                                        long long sx_out, int Kin, long long pin, long long pout,
+                                                        __bf16* __restrict__ out_hi, __bf16* __restrict__ out_lo) {
   extern __shared__ double smem_d[];
   T* Is = reinterpret_cast<T*>(smem_d);                 // [Kin][SP_LDI]
   int* io = reinterpret_cast<int*>(Is + Kin * SP_LDI);  // [Kin]
@@ -256,7 +314,12 @@ __global__ __launch_bounds__(256) void smallop_sparse_k(const T* __restrict__ in
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
   const int x = blockIdx.y, pl = blockIdx.z;
   in += (long long)x * sx_in + pl * pin;
-  out += (long long)x * sx_out + pl * pout;
+  if (SPLIT) {
+    out_hi += (long long)x * sx_out;
+    out_lo += (long long)x * sx_out;
+  } else {
+    out += (long long)x * sx_out + pl * pout;
+  }
   for (int i = tid; i < Kin; i += 256) io[i] = ioff[i];
   const int r0 = grp[wave], r1 = grp[wave + 1];
   T pre[SP_PRE];
@@ -287,7 +350,16 @@ __global__ __launch_bounds__(256) void smallop_sparse_k(const T* __restrict__ in
       T acc = T(0);
       const int k1 = rowptr[o + 1];
       for (int k = rowptr[o]; k < k1; ++k) acc += val[k] * Is[colidx[k] * SP_LDI + lane];
-      if (b < inner) out[(long long)ooff[o] + b] = acc;
+      if (b < inner) {
+        const long long off = (long long)ooff[o] + b;
+        if (SPLIT) {   // split-BF16x3 operand for the next GEMM: hi = bf16(acc), lo = bf16(acc - hi)
+          const __bf16 h = (__bf16)acc;
+          out_hi[off + pl * pout] = h;
+          out_lo[off + pl * pout] = (__bf16)(acc - (float)h);
+        } else {
+          out[off] = acc;
+        }
+      }
     }
   }
 }
@@ -395,23 +467,42 @@ __global__ __launch_bounds__(256) void smallop_mfma_k(const float* __restrict__ 
   }
 }
 
+bool smallop_sparse_ok(const SmallOp& s) {
+  return !s.wcplx && s.Kin <= SP_KMAX && 8LL * s.nnz <= (long long)s.Kin * s.Kout;
+}
+
+// The sparse pass on one real FP32 plane with its output split into bf16
+// (hi, lo) arrays (the A operand of the split-BF16x3 R-step GEMM).
+void run_smallop_split(Ctx* c, const SmallOp& s, const float* in, __bf16* hi, __bf16* lo, int X, int inner,
+                       long long sx_in, long long sx_out) {
+  if (X <= 0 || inner <= 0 || s.Kout <= 0) return;
+  DMRG_REQUIRE(X <= 65535, "smallop: outer extent too large");
+  DMRG_REQUIRE(smallop_sparse_ok(s), "smallop: split output needs the sparse real pass");
+  dim3 g((inner + SP_BN * SP_CHUNKS - 1) / (SP_BN * SP_CHUNKS), X, 1);
+  const size_t lds = sizeof(float) * (size_t)s.Kin * SP_LDI + sizeof(int) * s.Kin;
+  smallop_sparse_k<float, true><<<g, 256, lds, c->stream>>>(in, nullptr, s.rowptr, s.col, (const float*)s.val, s.grp,
+                                                            s.ioff, s.ooff, inner, sx_in, sx_out, s.Kin, 0, 0, hi,
+                                                            lo);
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
 void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, void* out, int X,
                  int inner, long long sx_in, long long sx_out, long long pin, long long pout) {
   if (X <= 0 || inner <= 0 || s.Kout <= 0) return;
   DMRG_REQUIRE(X <= 65535, "smallop: outer extent too large");
   const bool cv = field == CPLX;
-  if (!s.wcplx && s.Kin <= SP_KMAX && 8LL * s.nnz <= (long long)s.Kin * s.Kout) {
+  if (smallop_sparse_ok(s)) {
     dim3 g((inner + SP_BN * SP_CHUNKS - 1) / (SP_BN * SP_CHUNKS), X, cv ? 2 : 1);
     const size_t es = prec_size(prec);
     const size_t lds = es * (size_t)s.Kin * SP_LDI + sizeof(int) * s.Kin;
     if (prec == P64)
-      smallop_sparse_k<double><<<g, 256, lds, c->stream>>>((const double*)in, (double*)out, s.rowptr, s.col,
-                                                           (const double*)s.val, s.grp, s.ioff, s.ooff, inner, sx_in,
-                                                           sx_out, s.Kin, pin, pout);
+      smallop_sparse_k<double, false><<<g, 256, lds, c->stream>>>((const double*)in, (double*)out, s.rowptr, s.col,
+                                                                  (const double*)s.val, s.grp, s.ioff, s.ooff, inner,
+                                                                  sx_in, sx_out, s.Kin, pin, pout, nullptr, nullptr);
     else
-      smallop_sparse_k<float><<<g, 256, lds, c->stream>>>((const float*)in, (float*)out, s.rowptr, s.col,
-                                                          (const float*)s.val, s.grp, s.ioff, s.ooff, inner, sx_in,
-                                                          sx_out, s.Kin, pin, pout);
+      smallop_sparse_k<float, false><<<g, 256, lds, c->stream>>>((const float*)in, (float*)out, s.rowptr, s.col,
+                                                                 (const float*)s.val, s.grp, s.ioff, s.ooff, inner,
+                                                                 sx_in, sx_out, s.Kin, pin, pout, nullptr, nullptr);
     DMRG_CHECK_HIP(hipGetLastError());
     return;
   }
@@ -483,7 +574,8 @@ static double wget(const double* W, int i0, int i1, int i2, int i3, int n1, int 
 // T2 intermediate [(a' p')][(wr b)] and, when P*Dr is not a multiple of 4,
 // the contraction-precision copy of theta.
 static int pad4(long long v) { return (int)((v + 3) / 4 * 4); }
-static int t2_pitch(int wr, int Dr) { return pad4((long long)wr * Dr); }
+static int pad8(long long v) { return (int)((v + 7) / 8 * 8); }
+static int t2_pitch(int wr, int Dr) { return pad8((long long)wr * Dr); }
 
 HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   HeffOp h;
@@ -566,6 +658,23 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
     if (d.prec != P64) b += np_ * n * (8.0 + es) * 2.0;       // x to and y from FP64
     h.bytes_per_apply = b;
   }
+  if (d.prec == PB3 && d.field == REAL && smallop_sparse_ok(h.sop)) {
+    // environments split once per operator (they are fixed for the whole
+    // Krylov solve): L rows [(wl a')][a], R transposed to rows [b'][(wr b)]
+    const int ldL = d.ldL ? d.ldL : Dl, ldR = d.ldR ? d.ldR : Dr;
+    h.ldLb = pad8(Dl);
+    h.ldRb = pad8((long long)wr * Dr);
+    const size_t nL = (size_t)wl * Dl * h.ldLb, nR = (size_t)Dr * h.ldRb;
+    __bf16* bl = (__bf16*)ws_get(c, WS_B3_L, sizeof(__bf16) * 2 * nL);
+    __bf16* br = (__bf16*)ws_get(c, WS_B3_R, sizeof(__bf16) * 2 * nR);
+    split_rows<float>(c, (const float*)d.L, (long long)wl * Dl, Dl, ldL, bl, bl + nL, h.ldLb);
+    split_transpose<float>(c, (const float*)d.R, wr * Dr, Dr, ldR, br, br + nR, h.ldRb);
+    h.Lh = bl;
+    h.Ll = bl + nL;
+    h.RTh = br;
+    h.RTl = br + nR;
+    h.b3 = true;
+  }
   return h;
 }
 
@@ -583,6 +692,23 @@ double heff_flops(const dmrg_heff_desc& d) {
   return (d.field == CPLX ? 8.0 : 2.0) * mac;
 }
 
+// y += w sum_k p_k <p_k|x> for the penalty projectors of the operator (FP64;
+// the coefficients stay on the device)
+static void apply_projectors(Ctx* c, const HeffOp& h, const double* x, double* y) {
+  const dmrg_heff_desc& d = h.d;
+  if (d.nproj <= 0) return;
+  const int np = d.field == CPLX ? 2 : 1;
+  double* coef = (double*)ws_get(c, WS_PROJ, sizeof(double) * 2 * (size_t)d.nproj);
+  const long long ldp = h.n * np;
+  vec_dot_block(c, d.field, d.proj, ldp, d.nproj, x, h.n, coef);
+  scale_vec_k<<<1, 128, 0, c->stream>>>(coef, np * d.nproj, -d.proj_weight);
+  DMRG_CHECK_HIP(hipGetLastError());
+  vec_sub_block(c, d.field, d.proj, ldp, d.nproj, coef, y, h.n);
+}
+
+void run_smallop_split(Ctx* c, const SmallOp& s, const float* in, __bf16* hi, __bf16* lo, int X, int inner,
+                       long long sx_in, long long sx_out);
+
 // y = H_eff x, x and y FP64 (planar if complex), length h.n (per plane).
 void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
   const dmrg_heff_desc& d = h.d;
@@ -595,6 +721,24 @@ void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
   const long long nT = (long long)wl * Dl * P * Dr;   // T1 per plane
   const long long nT2 = (long long)Dl * P * ldT2;     // T2 per plane (rows pitched)
   ProfScope ps(c, 0, h.flops_per_apply, h.bytes_per_apply);
+  if (h.b3) {
+    // split-BF16x3 on pre-split operands: theta^T split, L-GEMM, MPO pass
+    // writing T2 split, R-GEMM, y back to FP64
+    const int ldxb = pad8(Dl);
+    const size_t nxt = (size_t)P * Dr * ldxb, nt2 = (size_t)Dl * P * ldT2;
+    __bf16* xt = (__bf16*)ws_get(c, WS_XIN, sizeof(__bf16) * 2 * nxt);
+    split_transpose<double>(c, x, Dl, P * Dr, P * Dr, xt, xt + nxt, ldxb);
+    float* T1f = (float*)ws_get(c, WS_T1, sizeof(float) * nT);
+    gemm_b3nt(c, wl * Dl, P * Dr, Dl, h.Lh, h.Ll, h.ldLb, xt, xt + nxt, ldxb, T1f, P * Dr);
+    __bf16* t2 = (__bf16*)ws_get(c, WS_T2, sizeof(__bf16) * 2 * nt2);
+    run_smallop_split(c, h.sop, T1f, t2, t2 + nt2, Dl, Dr, (long long)P * Dr, (long long)P * ldT2);
+    float* y32 = (float*)ws_get(c, WS_YOUT, sizeof(float) * n);
+    gemm_b3nt(c, Dl * P, Dr, wr * Dr, t2, t2 + nt2, ldT2, h.RTh, h.RTl, h.ldRb, y32, Dr);
+    from_prec(c, P32, y32, y, n);
+    apply_projectors(c, h, x, y);
+    c->matvecs++;
+    return;
+  }
   // theta as the L-GEMM operand: rows of P*Dr elements, pitched to a multiple
   // of 4 when needed; converted to the contraction precision on the way
   const int ldx = ((long long)P * Dr) % 4 == 0 ? P * Dr : pad4((long long)P * Dr);
@@ -621,15 +765,7 @@ void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
     gemm_prec(c, d.prec, g, T2, d.R, yout);
   }
   if (d.prec != P64) from_prec(c, d.prec, yout, y, n * np);
-  if (d.nproj > 0) {
-    // y += w sum_k p_k <p_k|x>  (FP64; coefficients stay on the device)
-    double* coef = (double*)ws_get(c, WS_PROJ, sizeof(double) * 2 * (size_t)d.nproj);
-    const long long ldp = n * np;
-    vec_dot_block(c, d.field, d.proj, ldp, d.nproj, x, n, coef);
-    scale_vec_k<<<1, 128, 0, c->stream>>>(coef, (d.field == CPLX ? 2 : 1) * d.nproj, -d.proj_weight);
-    DMRG_CHECK_HIP(hipGetLastError());
-    vec_sub_block(c, d.field, d.proj, ldp, d.nproj, coef, y, n);
-  }
+  apply_projectors(c, h, x, y);
   c->matvecs++;
 }
 

@@ -72,7 +72,7 @@ struct DevBuf {
 enum Slot {
   WS_T1 = 0, WS_T2, WS_XIN, WS_YOUT, WS_WSMALL, WS_ENV1, WS_ENV2,
   WS_KRYLOV_Q, WS_KRYLOV_W, WS_KRYLOV_RED, WS_KRYLOV_COEF,
-  WS_SVD_GRAM, WS_SVD_EV, WS_SVD_TMP, WS_SVD_TMP2, WS_SVD_INFO, WS_SVD_CPLX, WS_SVD_FIX, WS_SVD_RED, WS_PROJ,
+  WS_SVD_GRAM, WS_SVD_EV, WS_SVD_TMP, WS_SVD_TMP2, WS_SVD_INFO, WS_SVD_CPLX, WS_SVD_FIX, WS_SVD_RED, WS_PROJ, WS_B3_L, WS_B3_R,
   WS_BF16_A, WS_BF16_B, WS_REFINE,
   WS_NSLOTS
 };
@@ -126,5 +126,10 @@ void gemm_f64(Ctx* c, const GemmDesc& g, const double* A, const double* B, doubl
 void gemm_f32(Ctx* c, const GemmDesc& g, const float* A, const float* B, float* C);
 // split-BF16x3: FP32 operands split on the fly into bf16 hi/lo pairs.
 void gemm_bf16x3(Ctx* c, const GemmDesc& g, const float* A, const float* B, float* C);
+// split-BF16x3 on pre-split operands: C[M][N] = A[M][K] * B[N][K]^T, A and B
+// as (hi, lo) bf16 pairs with k-contiguous rows (pitches multiples of 8 and
+// >= K, 16-byte aligned); the pitch padding may hold anything (masked).
+void gemm_b3nt(Ctx* c, int M, int N, int K, const __bf16* Ah, const __bf16* Al, int lda, const __bf16* Bh,
+               const __bf16* Bl, int ldb, float* C, int ldc);
 
 }  // namespace dmrg

@@ -31,6 +31,11 @@ struct HeffOp {
   long long n = 0;      // vector length per plane
   double flops_per_apply = 0;
   double bytes_per_apply = 0;   // algorithmic HBM bytes (each operand read once, each result written once)
+  // split-BF16x3 on pre-split operands (real field): L as (hi, lo) rows
+  // [(wl a')][a] and R transposed as (hi, lo) rows [b'][(wr b)]
+  bool b3 = false;
+  const __bf16 *Lh = nullptr, *Ll = nullptr, *RTh = nullptr, *RTl = nullptr;
+  int ldLb = 0, ldRb = 0;
 };
 
 // Records a pair of HIP events around a region on the context stream when
@@ -55,6 +60,7 @@ SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& w
 void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, void* out, int X,
                  int inner, long long sx_in, long long sx_out, long long pin, long long pout);
 void gemm_prec(Ctx* c, int prec, const GemmDesc& g, const void* A, const void* B, void* C);
+bool smallop_sparse_ok(const SmallOp& s);
 GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag);
 
 double heff_flops(const dmrg_heff_desc& d);

@@ -531,6 +531,160 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_k(const float* __restrict__ A
 }
 
 // ---------------------------------------------------------------------------
+// split-BF16x3 kernel on pre-split operands: C[m][n] = sum_k A[m][k] B[n][k]
+// with A and B each given as a (hi, lo) pair of bf16 arrays whose rows are
+// k-contiguous (row pitches multiples of 8 elements, 16-byte aligned), so
+// every global load and every LDS store is a 16-byte vector and each MFMA
+// operand fragment (8 consecutive k) is one 16-byte LDS read.  The split
+// happens once per element outside the GEMM (environments once per bond,
+// theta once per apply, the MPO pass writes T2 split).  128x128 block tile,
+// BK = 32 (one v_mfma_f32_16x16x32_bf16 k-step), double-buffered LDS with
+// one barrier per k-step, XCD-aware tile order; C is FP32.
+// ---------------------------------------------------------------------------
+struct B3Args {
+  int M, N, K, lda, ldb, ldc;
+};
+
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void gemm_b3nt_k(const __bf16* __restrict__ Ah, const __bf16* __restrict__ Al,
+                                                   const __bf16* __restrict__ Bh, const __bf16* __restrict__ Bl,
+                                                   float* __restrict__ C, B3Args g) {
+  constexpr int BM = 32 * TM, BN = 32 * TN, BK = 32, LDK = BK + 8;   // row pitch 80 bytes
+  constexpr int NA = BM * BK / (8 * 256), NB = BN * BK / (8 * 256); // 16-byte vectors per thread
+  __shared__ __attribute__((aligned(16))) __bf16 sAh[2][BM][LDK], sAl[2][BM][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 sBh[2][BN][LDK], sBl[2][BN][LDK];
+  const int nbx = gridDim.x, nblk = gridDim.x * gridDim.y;
+  int lin = blockIdx.y * nbx + blockIdx.x;
+  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
+  const int m0 = (lin / nbx) * BM, n0 = (lin % nbx) * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int M = g.M, N = g.N, K = g.K;
+  flt4 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = flt4{0.f, 0.f, 0.f, 0.f};
+  bf16x8 rah[NA], ral[NA], rbh[NB], rbl[NB];
+  const bf16x8 zero8 = bf16x8{};
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      const int v = tid + 256 * e, r = v / (BK / 8), c = (v % (BK / 8)) * 8;
+      const int gm = m0 + r, gk = k0 + c;
+      if (gm < M && gk < K) {   // the row pitch covers the vector; k >= K is masked
+        const long long off = (long long)gm * g.lda + gk;
+        rah[e] = *reinterpret_cast<const bf16x8*>(Ah + off);
+        ral[e] = *reinterpret_cast<const bf16x8*>(Al + off);
+        if (gk + 8 > K) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if (gk + j >= K) { rah[e][j] = (__bf16)0.f; ral[e][j] = (__bf16)0.f; }
+        }
+      } else {
+        rah[e] = zero8;
+        ral[e] = zero8;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int v = tid + 256 * e, r = v / (BK / 8), c = (v % (BK / 8)) * 8;
+      const int gn = n0 + r, gk = k0 + c;
+      if (gn < N && gk < K) {
+        const long long off = (long long)gn * g.ldb + gk;
+        rbh[e] = *reinterpret_cast<const bf16x8*>(Bh + off);
+        rbl[e] = *reinterpret_cast<const bf16x8*>(Bl + off);
+        if (gk + 8 > K) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if (gk + j >= K) { rbh[e][j] = (__bf16)0.f; rbl[e][j] = (__bf16)0.f; }
+        }
+      } else {
+        rbh[e] = zero8;
+        rbl[e] = zero8;
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      const int v = tid + 256 * e, r = v / (BK / 8), c = (v % (BK / 8)) * 8;
+      *reinterpret_cast<bf16x8*>(&sAh[buf][r][c]) = rah[e];
+      *reinterpret_cast<bf16x8*>(&sAl[buf][r][c]) = ral[e];
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int v = tid + 256 * e, r = v / (BK / 8), c = (v % (BK / 8)) * 8;
+      *reinterpret_cast<bf16x8*>(&sBh[buf][r][c]) = rbh[e];
+      *reinterpret_cast<bf16x8*>(&sBl[buf][r][c]) = rbl[e];
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    const bool more = k0 + BK < K;
+    if (more) load(k0 + BK);
+    bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a) {
+      const int row = wm * TM * 16 + a * 16 + li;
+      ah[a] = *reinterpret_cast<const bf16x8*>(&sAh[cur][row][lk * 8]);
+      al[a] = *reinterpret_cast<const bf16x8*>(&sAl[cur][row][lk * 8]);
+    }
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+      const int col = wn * TN * 16 + b * 16 + li;
+      bh[b] = *reinterpret_cast<const bf16x8*>(&sBh[cur][col][lk * 8]);
+      bl[b] = *reinterpret_cast<const bf16x8*>(&sBl[cur][col][lk * 8]);
+    }
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      }
+    if (more) store(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * TM * 16 + a * 16 + lk * 4 + r;
+        const int col = n0 + wn * TN * 16 + b * 16 + li;
+        if (row < M && col < N) C[(long long)row * g.ldc + col] = acc[a][b][r];
+      }
+}
+
+void gemm_b3nt(Ctx* c, int M, int N, int K, const __bf16* Ah, const __bf16* Al, int lda, const __bf16* Bh,
+               const __bf16* Bl, int ldb, float* C, int ldc) {
+  DMRG_REQUIRE(M > 0 && N > 0 && K > 0, "gemm_b3nt: bad sizes");
+  DMRG_REQUIRE(lda % 8 == 0 && ldb % 8 == 0 && lda >= K && ldb >= K, "gemm_b3nt: pitches must be multiples of 8");
+  auto al = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  DMRG_REQUIRE(al(Ah) && al(Al) && al(Bh) && al(Bl), "gemm_b3nt: operands must be 16-byte aligned");
+  B3Args g{M, N, K, lda, ldb, ldc};
+  const long long tiles128 = (long long)((M + 127) / 128) * ((N + 127) / 128);
+  if (tiles128 >= 256) {
+    dim3 grid((N + 127) / 128, (M + 127) / 128);
+    DMRG_REQUIRE(grid.y <= 65535, "gemm_b3nt: M too large");
+    gemm_b3nt_k<4, 4><<<grid, 256, 0, c->stream>>>(Ah, Al, Bh, Bl, C, g);
+  } else {
+    dim3 grid((N + 63) / 64, (M + 63) / 64);
+    DMRG_REQUIRE(grid.y <= 65535, "gemm_b3nt: M too large");
+    gemm_b3nt_k<2, 2><<<grid, 256, 0, c->stream>>>(Ah, Al, Bh, Bl, C, g);
+  }
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // host dispatch
 // ---------------------------------------------------------------------------
 static GemmArgs make_args(const GemmDesc& d) {

@@ -264,3 +264,27 @@ def test_excited_states_vs_ed():
     for a in range(3):
         for b in range(a):
             assert abs(alg.overlap(ch.eng, wfs[a], wfs[b])) < 1e-6
+
+
+def test_generalized_eigenvalue_vs_dense():
+    """gs_energy_generalized (chain.py:230-251): smallest lambda of
+    H psi = lambda A psi with A = 1 + sum_i c_i Sz_i^2-like positive terms,
+    against scipy.linalg.eigh(H, A) on the dense matrices (N=8)."""
+    from scipy.linalg import eigh
+    from dmrgpy_amd.chain import Chain
+    from dmrgpy_amd.multioperator import heisenberg_terms
+    from oracle import ed
+    n = 8
+    rng = np.random.default_rng(29)
+    terms_h = heisenberg_terms(n)
+    terms_a = [(1.0, [("Id", 1)])] + [(float(c), [("Sz", i + 1), ("Sz", i + 1)])
+                                     for i, c in enumerate(rng.uniform(0.2, 0.8, n))]
+    H = ed.sparse_hamiltonian([2] * n, terms_h).toarray()
+    A = ed.sparse_hamiltonian([2] * n, terms_a).toarray()
+    lam_ref = eigh(H, A, eigvals_only=True)[0]
+    ch = Chain([2] * n, field="real")
+    ch.set_sweep_params(16, 12, 1e-12, 0.0)
+    ch.set_hamiltonian(terms_h)
+    np.random.seed(5)
+    lam = ch.gs_energy_generalized(terms_a)
+    assert abs(lam - lam_ref) < 1e-9, (lam, lam_ref)
