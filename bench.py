@@ -122,6 +122,7 @@ def run_sweeps(ch, psi, K, prec_name, maxdim):
             "heff_bytes": nbytes[0],
             "env_ms": tms[1], "svd_ms": tms[2], "svd_calls": calls[2],
             "iters_per_bond": sum(stats.lanczos_iters) / max(1, len(stats.lanczos_iters)),
+            "svd_guided_fraction": sum(stats.svd_guided) / max(1, len(stats.svd_guided)),
             "max_bond": max(psi.bond_dims())}
 
 
@@ -352,6 +353,11 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--svd", default="full", choices=["guided", "full"],
+                    help="two-site splits: the full split (the reference's rule), or the subspace split guided "
+                         "by the neighbouring isometry (falls back to the full split when it misses weight)")
+    ap.add_argument("--no-full-svd-check", action="store_true",
+                    help="skip the timed sweep with the full split from the same state (guided runs only)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--workload", default="dmrg", choices=["dmrg", "small", "tdvp", "gradients"],
                     help="dmrg: the headline sweep benchmark; tdvp / gradients: the TDVP-quench and "
@@ -393,6 +399,7 @@ def main():
     ramp = [m for m in spec["ramp"] if m < D]
     psi = ch._default_mps(maxm=ramp[0] if ramp else D)
     eng = ch.eng
+    eng.svd_guided = args.svd == "guided"
     t_setup = time.perf_counter()
     prec = PREC_NAMES[args.precision]
     for m in ramp:
@@ -406,6 +413,12 @@ def main():
     start = psi.copy()
     res = run_sweeps(ch, psi, args.steps, args.precision, D)
     log("timed {}: {}".format(args.precision, res))
+    resfull = None
+    if args.svd == "guided" and not args.no_full_svd_check:
+        eng.svd_guided = False
+        resfull = run_sweeps(ch, start.copy(), args.steps, args.precision, D)
+        eng.svd_guided = True
+        log("timed {} with the full split: {}".format(args.precision, resfull))
     res64 = None
     if not args.no_fp64:
         psi64 = start.copy()
@@ -420,7 +433,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(D, spec["d"], spec["w"], n, res["iters_per_bond"], args.cpu_threads)
         log("cpu baseline: {}".format(cpu))
-    achieved = flops_per / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    # split-BF16x3 runs three BF16 MFMA products per FP32-equivalent flop:
+    # priced against the BF16 peak with the products it actually issues
+    mfma_mult = 3.0 if args.precision == "bf16x3" else 1.0
+    achieved = mfma_mult * flops_per / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     traffic = measured_traffic(args.model, D, args.precision)
     out = {
         "metric": METRIC,
@@ -439,11 +455,12 @@ def main():
                                "{} contractions".format(D, spec["desc"].format(n=n), ramp, args.warmup,
                                                         args.precision),
                    "model": args.model, "N": n, "D": D, "d": spec["d"], "w": spec["w"], "cutoff": 1e-12,
-                   "precision": args.precision, "field": "real"},
+                   "precision": args.precision, "field": "real", "svd": args.svd},
         "max_bond": res["max_bond"],
         "heff_tflops": heff_tflops,
         "heff_calls": res["heff_calls"],
         "lanczos_iters_per_bond": res["iters_per_bond"],
+        "svd_guided_fraction": res["svd_guided_fraction"],
         "sweep_s": res["sweep_s"],
         "time_split_s_per_sweep": {"heff": res["heff_ms"] / 1e3 / args.steps,
                                    "env": res["env_ms"] / 1e3 / args.steps,
@@ -453,8 +470,17 @@ def main():
                      "frac": achieved / peak, "traffic": traffic,
                      "algorithmic_bytes": res["heff_bytes"] / max(1, res["heff_calls"]),
                      "kernel": "H_eff two-site apply (L-GEMM, W small-op, R-GEMM), {:.3g} GFLOP per apply "
-                               "averaged over the sweep's bonds, {:.3f} ms average".format(flops_per / 1e9, avg_ms)},
+                               "averaged over the sweep's bonds{}, {:.3f} ms average".format(
+                                   flops_per / 1e9, " (x3 BF16 products)" if mfma_mult > 1 else "", avg_ms)},
     }
+    if resfull is not None:
+        out["full_svd"] = {"sweep_s": resfull["sweep_s"], "energy": resfull["energy"],
+                           "time_split_s_per_sweep": {"heff": resfull["heff_ms"] / 1e3 / args.steps,
+                                                      "env": resfull["env_ms"] / 1e3 / args.steps,
+                                                      "svd": resfull["svd_ms"] / 1e3 / args.steps},
+                           "max_bond": resfull["max_bond"],
+                           "dE_per_site_vs_guided": abs(res["energy"] - resfull["energy"]) / n}
+        out["speedup_guided_vs_full_svd"] = resfull["sweep_s"] / res["sweep_s"]
     if res64 is not None:
         out["fp64"] = {"sweep_s": res64["sweep_s"], "energy": res64["energy"],
                        "heff_tflops": res64["heff_flops"] / (res64["heff_ms"] * 1e-3) / 1e12,

@@ -53,6 +53,9 @@ SIGNATURES = {
                                  c_int, c_double, c_double, P_int]),
     "dmrg_svd_split": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_int, c_int, c_int,
                                c_void_p, c_void_p, c_int, P_int, P_double, P_double]),
+    "dmrg_svd_split_guided": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_double, c_int, c_int, c_int,
+                                      c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, P_int, P_double,
+                                      P_double, P_int, P_double]),
     "dmrg_mpo_grad": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
                               c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p]),
     "dmrg_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,

@@ -38,6 +38,8 @@ class Stats:
         self.lanczos_iters = []
         self.krylov_iters = []
         self.truncation = []      # (bond, keep, discarded weight)
+        self.svd_guided = []      # 1 where the guided subspace split produced the result
+        self.svd_resid = []       # its measured weight outside the subspace (-1: not attempted)
 
     def reset(self):
         self.__init__()
@@ -115,15 +117,26 @@ def local_ground(eng, L, R, mpo, psi, i, niter, prec, stats=None, tol=LANCZOS_TO
 
 
 def update(eng, psi, i, th, dims, cutoff, maxdim, direction, stats=None):
+    """Split theta back into sites i, i+1 (dmrg.py:336-352).  In a sweep the
+    site the split leaves behind was an isometry before the update (A_{i+1}
+    right-orthonormal when moving right, A_i left-orthonormal when moving
+    left) whose span holds almost all of theta; it guides the subspace split
+    when eng.svd_guided is set (the library falls back to the full split
+    whenever the subspace misses weight)."""
     Dl, d1, d2, Dr = dims
     npl = th.shape[0]
+    guide, side = None, 0
+    if getattr(eng, "svd_guided", False):
+        guide, side = (psi.A(i + 1), 0) if direction == "right" else (psi.A(i), 1)
     U, V, keep, disc, _ = eng.svd_split(psi.field, th, Dl * d1, d2 * Dr, cutoff, maxdim, 1,
-                                        0 if direction == "right" else 1)
+                                        0 if direction == "right" else 1, guide=guide, guide_side=side)
     psi.set_A(i, U.view(npl, Dl, d1, keep))
     psi.set_A(i + 1, V.view(npl, keep, d2, Dr))
     psi.center = i + 1 if direction == "right" else i
     if stats is not None:
         stats.truncation.append((i, keep, disc))
+        stats.svd_guided.append(eng.last_svd_guided)
+        stats.svd_resid.append(eng.last_svd_resid)
     return keep, disc
 
 

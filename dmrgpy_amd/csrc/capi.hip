@@ -19,6 +19,10 @@ void krylov_expm(Ctx* c, const HeffOp& h, double* x, std::complex<double> coeff,
 double rayleigh(Ctx* c, const HeffOp& h, const double* x);
 int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
                     int absorb, double* U, double* V, int kcap, int* keep, double* discarded, double* svals_host);
+int svd_split_guided_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim,
+                           int mindim, int absorb, const double* G, int k0, int gside, int sketch, double* U,
+                           double* V, int kcap, int* keep, double* discarded, double* svals_host, int* used,
+                           double* resid);
 void mpo_grad(Ctx* c, int field, int prec, const void* L, const double* bra64, const double* ket64, const void* R,
               int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, int ldL, int ldR, double* G_host);
 
@@ -199,6 +203,19 @@ int dmrg_svd_split(dmrg_ctx* ctx, int field, const double* M, int m, int n, doub
   DMRG_REQUIRE(field == 0 || field == 1, "svd_split: bad field");
   dmrg::svd_split_entry(&ctx->c, field, M, m, n, cutoff, maxdim, mindim, absorb, U, V, kcap, keep, discarded,
                         svals_host);
+  API_END
+}
+
+int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int n, double cutoff, int maxdim,
+                          int mindim, int absorb, const double* G, int k0, int gside, int sketch, double* U,
+                          double* V, int kcap, int* keep, double* discarded, double* svals_host, int* used,
+                          double* resid) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && M && U && V && keep, "svd_split_guided: null argument");
+  DMRG_REQUIRE(field == 0 || field == 1, "svd_split_guided: bad field");
+  DMRG_REQUIRE(k0 >= 0 && sketch >= 0, "svd_split_guided: negative guide size");
+  dmrg::svd_split_guided_entry(&ctx->c, field, M, m, n, cutoff, maxdim, mindim, absorb, G, k0, gside, sketch, U, V,
+                               kcap, keep, discarded, svals_host, used, resid);
   API_END
 }
 

@@ -729,11 +729,11 @@ void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
     __bf16* xt = (__bf16*)ws_get(c, WS_XIN, sizeof(__bf16) * 2 * nxt);
     split_transpose<double>(c, x, Dl, P * Dr, P * Dr, xt, xt + nxt, ldxb);
     float* T1f = (float*)ws_get(c, WS_T1, sizeof(float) * nT);
-    gemm_b3nt(c, wl * Dl, P * Dr, Dl, h.Lh, h.Ll, h.ldLb, xt, xt + nxt, ldxb, T1f, P * Dr);
+    gemm_b3nt(c, wl * Dl, P * Dr, Dl, h.Lh, h.Ll, h.ldLb, xt, xt + nxt, ldxb, T1f, P * Dr, 1);
     __bf16* t2 = (__bf16*)ws_get(c, WS_T2, sizeof(__bf16) * 2 * nt2);
     run_smallop_split(c, h.sop, T1f, t2, t2 + nt2, Dl, Dr, (long long)P * Dr, (long long)P * ldT2);
     float* y32 = (float*)ws_get(c, WS_YOUT, sizeof(float) * n);
-    gemm_b3nt(c, Dl * P, Dr, wr * Dr, t2, t2 + nt2, ldT2, h.RTh, h.RTl, h.ldRb, y32, Dr);
+    gemm_b3nt(c, Dl * P, Dr, wr * Dr, t2, t2 + nt2, ldT2, h.RTh, h.RTl, h.ldRb, y32, Dr, 2);
     from_prec(c, P32, y32, y, n);
     apply_projectors(c, h, x, y);
     c->matvecs++;

@@ -74,6 +74,7 @@ enum Slot {
   WS_KRYLOV_Q, WS_KRYLOV_W, WS_KRYLOV_RED, WS_KRYLOV_COEF,
   WS_SVD_GRAM, WS_SVD_EV, WS_SVD_TMP, WS_SVD_TMP2, WS_SVD_INFO, WS_SVD_CPLX, WS_SVD_FIX, WS_SVD_RED, WS_PROJ, WS_B3_L, WS_B3_R,
   WS_BF16_A, WS_BF16_B, WS_REFINE,
+  WS_GSV_Q, WS_GSV_B, WS_GSV_R, WS_GSV_S, WS_GSV_V, WS_GSV_T, WS_GSV_T2, WS_GSV_OU, WS_GSV_OV,
   WS_NSLOTS
 };
 
@@ -129,7 +130,8 @@ void gemm_bf16x3(Ctx* c, const GemmDesc& g, const float* A, const float* B, floa
 // split-BF16x3 on pre-split operands: C[M][N] = A[M][K] * B[N][K]^T, A and B
 // as (hi, lo) bf16 pairs with k-contiguous rows (pitches multiples of 8 and
 // >= K, 16-byte aligned); the pitch padding may hold anything (masked).
+// tag as in GemmDesc (1 H_eff L-step, 2 R-step).
 void gemm_b3nt(Ctx* c, int M, int N, int K, const __bf16* Ah, const __bf16* Al, int lda, const __bf16* Bh,
-               const __bf16* Bl, int ldb, float* C, int ldc);
+               const __bf16* Bl, int ldb, float* C, int ldc, int tag);
 
 }  // namespace dmrg

@@ -545,7 +545,7 @@ struct B3Args {
   int M, N, K, lda, ldb, ldc;
 };
 
-template <int TM, int TN>
+template <int TM, int TN, int TAG>
 __global__ __launch_bounds__(256) void gemm_b3nt_k(const __bf16* __restrict__ Ah, const __bf16* __restrict__ Al,
                                                    const __bf16* __restrict__ Bh, const __bf16* __restrict__ Bl,
                                                    float* __restrict__ C, B3Args g) {
@@ -664,8 +664,16 @@ __global__ __launch_bounds__(256) void gemm_b3nt_k(const __bf16* __restrict__ Ah
       }
 }
 
+template <int TM, int TN>
+static void launch_b3nt(Ctx* c, dim3 grid, int tag, const __bf16* Ah, const __bf16* Al, const __bf16* Bh,
+                        const __bf16* Bl, float* C, const B3Args& g) {
+  if (tag == 1) gemm_b3nt_k<TM, TN, 1><<<grid, 256, 0, c->stream>>>(Ah, Al, Bh, Bl, C, g);
+  else if (tag == 2) gemm_b3nt_k<TM, TN, 2><<<grid, 256, 0, c->stream>>>(Ah, Al, Bh, Bl, C, g);
+  else gemm_b3nt_k<TM, TN, 0><<<grid, 256, 0, c->stream>>>(Ah, Al, Bh, Bl, C, g);
+}
+
 void gemm_b3nt(Ctx* c, int M, int N, int K, const __bf16* Ah, const __bf16* Al, int lda, const __bf16* Bh,
-               const __bf16* Bl, int ldb, float* C, int ldc) {
+               const __bf16* Bl, int ldb, float* C, int ldc, int tag) {
   DMRG_REQUIRE(M > 0 && N > 0 && K > 0, "gemm_b3nt: bad sizes");
   DMRG_REQUIRE(lda % 8 == 0 && ldb % 8 == 0 && lda >= K && ldb >= K, "gemm_b3nt: pitches must be multiples of 8");
   auto al = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
@@ -675,11 +683,11 @@ void gemm_b3nt(Ctx* c, int M, int N, int K, const __bf16* Ah, const __bf16* Al, 
   if (tiles128 >= 256) {
     dim3 grid((N + 127) / 128, (M + 127) / 128);
     DMRG_REQUIRE(grid.y <= 65535, "gemm_b3nt: M too large");
-    gemm_b3nt_k<4, 4><<<grid, 256, 0, c->stream>>>(Ah, Al, Bh, Bl, C, g);
+    launch_b3nt<4, 4>(c, grid, tag, Ah, Al, Bh, Bl, C, g);
   } else {
     dim3 grid((N + 63) / 64, (M + 63) / 64);
     DMRG_REQUIRE(grid.y <= 65535, "gemm_b3nt: M too large");
-    gemm_b3nt_k<2, 2><<<grid, 256, 0, c->stream>>>(Ah, Al, Bh, Bl, C, g);
+    launch_b3nt<2, 2>(c, grid, tag, Ah, Al, Bh, Bl, C, g);
   }
   DMRG_CHECK_HIP(hipGetLastError());
 }

@@ -186,8 +186,8 @@ __global__ void set_identity_k(double* __restrict__ T, int k, long long plane, i
 // Z <- T Z  from Y = O, Z = I converges quadratically to Y = O^{1/2},
 // Z = O^{-1/2}.  X becomes Z X (X Z), the closest matrix with orthonormal
 // rows (columns), and the partner factor P becomes P Y (Y P), so the
-// product of the two factors is unchanged.  Returns false, with nothing
-// modified, when the iteration does not converge.
+// product of the two factors is unchanged (P may be null).  Returns false,
+// with nothing modified, when the iteration does not converge.
 static bool lowdin(Ctx* c, int field, double* X, int keep, int len, bool rows, double* P, int plen) {
   const bool cx = field == CPLX;
   const size_t es = cx ? 2 : 1;
@@ -238,13 +238,14 @@ static bool lowdin(Ctx* c, int field, double* X, int keep, int len, bool rows, d
     std::swap(Z, Zn);
   }
   if (!ok) return false;
-  const long long big = std::max((long long)len, (long long)plen) * keep;
+  const long long big = std::max((long long)len, (long long)(P ? plen : 0)) * keep;
   double* tmp = (double*)ws_get(c, WS_SVD_GRAM, sizeof(double) * es * (size_t)big);
   if (rows) {  // X <- Z X (keep x len), P <- P Y (plen x keep)
     GemmDesc d = gdesc(keep, len, keep, keep, len, len, field, 4);
     d.pA = kk; d.pB = pX; d.pC = pX;
     gemm_f64(c, d, Z, X, tmp);
     DMRG_CHECK_HIP(hipMemcpyAsync(X, tmp, sizeof(double) * es * pX, hipMemcpyDeviceToDevice, c->stream));
+    if (!P) return true;
     GemmDesc e = gdesc(plen, keep, keep, keep, keep, keep, field, 4);
     e.pA = pP; e.pB = kk; e.pC = pP;
     gemm_f64(c, e, P, Y, tmp);
@@ -254,6 +255,7 @@ static bool lowdin(Ctx* c, int field, double* X, int keep, int len, bool rows, d
     d.pA = pX; d.pB = kk; d.pC = pX;
     gemm_f64(c, d, X, Z, tmp);
     DMRG_CHECK_HIP(hipMemcpyAsync(X, tmp, sizeof(double) * es * pX, hipMemcpyDeviceToDevice, c->stream));
+    if (!P) return true;
     GemmDesc e = gdesc(keep, plen, keep, keep, plen, plen, field, 4);
     e.pA = kk; e.pB = pP; e.pC = pP;
     gemm_f64(c, e, Y, P, tmp);
@@ -262,11 +264,15 @@ static bool lowdin(Ctx* c, int field, double* X, int keep, int len, bool rows, d
   return true;
 }
 
-// The reference's keep rule on a descending spectrum.
-int truncate_rule(const std::vector<double>& s, double cutoff, int maxdim, int mindim, double* discarded) {
+// The reference's keep rule on a descending spectrum.  `extra` is weight
+// (in the units of s^2) known to lie outside the listed values -- the part
+// of a matrix a subspace split does not capture; it counts towards the
+// total and is always discarded (0 for an exact spectrum).
+int truncate_rule(const std::vector<double>& s, double cutoff, int maxdim, int mindim, double* discarded,
+                  double extra) {
   const int n = (int)s.size();
   std::vector<double> p(n);
-  double total = 0;
+  double total = extra;
   for (int i = 0; i < n; ++i) {
     p[i] = s[i] * s[i];
     total += p[i];
@@ -276,6 +282,7 @@ int truncate_rule(const std::vector<double>& s, double cutoff, int maxdim, int m
     return std::max(1, mindim);
   }
   for (int i = 0; i < n; ++i) p[i] /= total;
+  const double pextra = extra / total;
   const int hi = maxdim > 0 ? std::min(maxdim, n) : n;
   const int j_upper = std::max(0, n - mindim);
   const int j_forced = std::min(std::max(0, n - hi), j_upper);
@@ -285,7 +292,7 @@ int truncate_rule(const std::vector<double>& s, double cutoff, int maxdim, int m
   } else {
     // tail(t) = sum of the j_forced + t smallest, t = 0..j_upper-j_forced;
     // keep the largest t with tail <= cutoff (ascending prefix scan)
-    double forced = 0;
+    double forced = pextra;
     for (int t = 0; t < j_forced; ++t) forced += p[n - 1 - t];
     int best = 0;
     double tail = forced;
@@ -302,7 +309,7 @@ int truncate_rule(const std::vector<double>& s, double cutoff, int maxdim, int m
   }
   const int keep = n - j;
   if (discarded) {
-    double dsc = 0;
+    double dsc = pextra;
     for (int i = keep; i < n; ++i) dsc += p[i];
     *discarded = dsc;
   }
@@ -312,7 +319,11 @@ int truncate_rule(const std::vector<double>& s, double cutoff, int maxdim, int m
 struct SvdOut {
   int keep;
   double discarded;
+  double resid = -1.0;   // guided split: weight outside the subspace / |M|^2 (-1: not computed)
 };
+
+SvdOut svd_split(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
+                 int absorb, double* U, double* V, int kcap, std::vector<double>& svals);
 
 static SvdOut svd_full(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
                        int absorb, double* U, double* V, int kcap, std::vector<double>& svals) {
@@ -358,7 +369,7 @@ static SvdOut svd_full(Ctx* c, int field, const double* M, int m, int n, double 
   for (auto& s : svals) s *= scale;
   DMRG_CHECK_HIP(hipMemcpyAsync(S, svals.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
   SvdOut o;
-  o.keep = truncate_rule(svals, cutoff, maxdim, mindim, &o.discarded);
+  o.keep = truncate_rule(svals, cutoff, maxdim, mindim, &o.discarded, 0.0);
   DMRG_REQUIRE(o.keep <= kcap, "svd: keep exceeds output capacity");
   const int keep = o.keep;
   // U_M (row-major m x k) is the Vx buffer, Vh_M (k x n) is the Ux buffer.
@@ -389,7 +400,8 @@ static SvdOut svd_full(Ctx* c, int field, const double* M, int m, int n, double 
 // does.  A forced side larger than min(m, n) keeps the min(m, n) largest
 // eigenvalues (the rest are rounding noise of a rank-deficient Gram).
 static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mind,
-                       int absorb, double* U, double* V, int kcap, std::vector<double>& svals, int side) {
+                       int absorb, double* U, double* V, int kcap, std::vector<double>& svals, int side,
+                       double extra = 0.0) {
   const bool cx = field == CPLX;
   const bool left = side < 0 ? m <= n : side == 0;
   const int g = left ? m : n;
@@ -444,7 +456,7 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
   svals.resize(rank);
   for (int i = 0; i < rank; ++i) svals[i] = scale * std::sqrt(std::max(evh[g - 1 - i], 0.0));
   SvdOut o;
-  o.keep = truncate_rule(svals, cutoff, maxdim, mind, &o.discarded);
+  o.keep = truncate_rule(svals, cutoff, maxdim, mind, &o.discarded, extra);
   const int keep = o.keep;
   if (!(svals[0] > 0.0 && svals[keep - 1] >= 1e-7 * svals[0]))
     return svd_full(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals);
@@ -501,9 +513,302 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
     else if (!left && absorb == 0) fixed = lowdin(c, field, U, keep, m, false, V, n);
     // sk / E must outlive the async copies and kernels above
     DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
-    if (!fixed) return svd_gram(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals, left ? 1 : 0);
+    if (!fixed)
+      return svd_gram(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals, left ? 1 : 0, extra);
   }
   return o;
+}
+
+// ---------------------------------------------------------------------------
+// Subspace-guided split.  In a sweep the two-site matrix of bond i is
+// theta = A_i A_{i+1} with one factor an isometry from the previous sweep
+// (A_{i+1} with orthonormal rows when moving right, A_i with orthonormal
+// columns when moving left), and the local update changes theta only a
+// little.  So the rows of theta lie almost entirely in the row space of the
+// guide Q0 (k0 x n).  The split:
+//   1. sketch the part outside Q0: Y = Omega M (p x n, Omega uniform random),
+//      projected off Q0 twice; orthonormal rows Q1 from the eigenvectors of
+//      Y Y^dag (directions below 1e-7 of the largest dropped);
+//   2. Q = [Q0; Q1] re-orthonormalised (Loewdin), B = M Q^dag (m x l);
+//   3. residual weight r = |M - B Q|^2 -- exactly the weight the subspace
+//      misses; accepted only when r <= 0.1 * cutoff * |M|^2, or, when maxdim
+//      forces the truncation, r <= 0.1 * the weight maxdim discards;
+//      otherwise the caller runs the full split;
+//   4. the Gram-route split of B (l x l eigenproblem instead of min(m, n)),
+//      with r counted as discarded weight in the keep rule, and V = Vb Q.
+// The result is the exact truncated SVD of M Q^dag Q, which differs from
+// M by less than a tenth of the weight the keep rule discards anyway.  At a
+// cutoff-limited bond the weight an update moves outside the old span is
+// typically of the order of the cutoff itself (measured 1e-12..2e-11 at
+// cutoff 1e-12 on a D=1024 Hubbard chain, spread over hundreds of
+// directions), so there the full split runs; the guided split pays where
+// maxdim limits the bond or the cutoff is looser.
+// ---------------------------------------------------------------------------
+
+// uniform entries in [-1, 1) from a counter hash (splitmix64 finaliser)
+__global__ void sketch_k(double* __restrict__ O, long long n, unsigned long long seed) {
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  for (; i < n; i += (long long)gridDim.x * 256) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    O[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+// sum of squares, two passes in a fixed order (deterministic)
+__global__ __launch_bounds__(256) void sumsq_partial_k(const double* __restrict__ x, long long n,
+                                                       double* __restrict__ part) {
+  __shared__ double red[256];
+  double a = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    a += x[i] * x[i];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void sum_final_k(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double red[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) a += part[i];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+static void dev_sumsq(Ctx* c, const double* x, long long n, double* out) {
+  long long nb = (n + 4095) / 4096;
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  double* part = (double*)ws_get(c, WS_SVD_RED, sizeof(double) * 1024);
+  sumsq_partial_k<<<(unsigned)nb, 256, 0, c->stream>>>(x, n, part);
+  DMRG_CHECK_HIP(hipGetLastError());
+  sum_final_k<<<1, 256, 0, c->stream>>>(part, (int)nb, out);
+  DMRG_CHECK_HIP(hipGetLastError());
+}
+
+// planar block copy of `rows` rows of length `cols` (per plane)
+static void copy_rows(Ctx* c, bool cx, const double* src, long long psrc, double* dst, long long pdst,
+                      long long count) {
+  DMRG_CHECK_HIP(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToDevice, c->stream));
+  if (cx)
+    DMRG_CHECK_HIP(
+        hipMemcpyAsync(dst + pdst, src + psrc, sizeof(double) * count, hipMemcpyDeviceToDevice, c->stream));
+}
+
+// Row-guided split of M (m x n) with guide rows Q0 (k0 x n).  Returns false
+// when the subspace misses too much weight (nothing written that the caller
+// relies on).
+static bool svd_rowguided(Ctx* c, int field, const double* M, int m, int n, const double* Q0, int k0, int sketch,
+                          double cutoff, int maxdim, int mind, int absorb, double* U, double* V, int kcap,
+                          std::vector<double>& svals, SvdOut& out, double* resid) {
+  const bool cx = field == CPLX;
+  const size_t es = cx ? 2 : 1;
+  const long long mn = (long long)m * n;
+  const int p = std::min(sketch, n - k0);
+  if (p <= 0 || k0 <= 0) return false;
+  // scratch: Omega (p x m), Y (p x n), C (p x k0), Gy (p x p, interleaved if cx), E (p x p), Y1 (p x n)
+  const long long nO = (long long)p * m, nY = (long long)p * n, nC = (long long)p * k0, nG = (long long)p * p;
+  double* S = (double*)ws_get(c, WS_GSV_S, sizeof(double) * (es * (nO + 2 * nY + nC + 2 * nG) + 2 * (size_t)p + 64));
+  double* Om = S;
+  double* Y = Om + es * nO;
+  double* Cc = Y + es * nY;
+  double* Gy = Cc + es * nC;
+  double* E = Gy + es * nG;
+  double* Y1 = E + es * nG;
+  double* ev = Y1 + es * nY;
+  double* scal = ev + p;   // device scalars: [0] |R|^2, [1] |M|^2
+  rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
+  {
+    ProfScope ps(c, 2, (cx ? 8.0 : 2.0) * (double)p * ((double)m * n + 4.0 * n * k0 + (double)p * n));
+    sketch_k<<<g1(nO), 256, 0, c->stream>>>(Om, nO, 0x5EEDC0FFEEull);
+    DMRG_CHECK_HIP(hipGetLastError());
+    if (cx) DMRG_CHECK_HIP(hipMemsetAsync(Om + nO, 0, sizeof(double) * nO, c->stream));
+    {  // Y = Omega M
+      GemmDesc d = gdesc(p, n, m, m, n, n, field, 4);
+      d.pA = nO; d.pB = mn; d.pC = nY;
+      gemm_f64(c, d, Om, M, Y);
+    }
+    for (int pass = 0; pass < 2; ++pass) {  // Y <- Y - (Y Q0^dag) Q0
+      GemmDesc d = gdesc(p, k0, n, n, n, k0, field, 4);
+      d.transB = true; d.conjB = true;
+      d.pA = nY; d.pB = (long long)k0 * n; d.pC = nC;
+      gemm_f64(c, d, Y, Q0, Cc);
+      GemmDesc e = gdesc(p, n, k0, k0, n, n, field, 4);
+      e.alpha = -1.0; e.beta = 1.0;
+      e.pA = nC; e.pB = (long long)k0 * n; e.pC = nY;
+      gemm_f64(c, e, Cc, Q0, Y);
+    }
+    {  // Gy = Y Y^dag
+      GemmDesc d = gdesc(p, p, n, n, n, p, field, 4);
+      d.transB = true; d.conjB = true;
+      d.pA = d.pB = nY; d.pC = nG;
+      gemm_f64(c, d, Y, Y, E);
+    }
+    if (cx) {
+      planar_to_inter_k<<<g1(nG), 256, 0, c->stream>>>(E, nG, Gy);
+      DMRG_CHECK_HIP(hipGetLastError());
+      check_rb(rocsolver_zheevd(c->blas, rocblas_evect_original, rocblas_fill_upper, p, (rocblas_double_complex*)Gy,
+                                p, ev, Y1, info),
+               "zheevd");
+    } else {
+      DMRG_CHECK_HIP(hipMemcpyAsync(Gy, E, sizeof(double) * nG, hipMemcpyDeviceToDevice, c->stream));
+      check_rb(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, p, Gy, p, ev, Y1, info),
+               "dsyevd");
+    }
+  }
+  std::vector<double> evh(p);
+  int hinfo = 0;
+  DMRG_CHECK_HIP(hipMemcpyAsync(evh.data(), ev, sizeof(double) * p, hipMemcpyDeviceToHost, c->stream));
+  DMRG_CHECK_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  if (hinfo != 0) return false;
+  const double evmax = evh[p - 1];
+  int r = 0;
+  if (evmax > 0.0)
+    while (r < p && evh[p - 1 - r] > 1e-14 * evmax) ++r;
+  const int l = k0 + r;
+  const long long nQ = (long long)l * n;
+  double* Q = (double*)ws_get(c, WS_GSV_Q, sizeof(double) * es * (size_t)nQ);
+  copy_rows(c, cx, Q0, (long long)k0 * n, Q, nQ, (long long)k0 * n);
+  if (r > 0) {
+    // Q1 = diag(ev^-1/2) E Y with E the top-r eigenvector rows (descending)
+    std::vector<double> inv(r);
+    for (int i = 0; i < r; ++i) inv[i] = std::sqrt(evh[p - 1 - i]);
+    double* sdev = ev;   // ev already read back
+    DMRG_CHECK_HIP(hipMemcpyAsync(sdev, inv.data(), sizeof(double) * r, hipMemcpyHostToDevice, c->stream));
+    const long long nE = (long long)r * p;
+    if (cx) take_rows_k<true><<<g1(nE), 256, 0, c->stream>>>(Gy, p, p - 1, r, p, E, nE);
+    else take_rows_k<false><<<g1(nE), 256, 0, c->stream>>>(Gy, p, p - 1, r, p, E, nE);
+    DMRG_CHECK_HIP(hipGetLastError());
+    GemmDesc d = gdesc(r, n, p, p, n, n, field, 4);
+    d.pA = nE; d.pB = nY; d.pC = nQ;
+    gemm_f64(c, d, E, Y, Q + (long long)k0 * n);
+    if (cx) scale_k<true><<<g1((long long)r * n), 256, 0, c->stream>>>(Q + (long long)k0 * n, r, n, nQ, sdev, 1, 1);
+    else scale_k<false><<<g1((long long)r * n), 256, 0, c->stream>>>(Q + (long long)k0 * n, r, n, 0, sdev, 1, 1);
+    DMRG_CHECK_HIP(hipGetLastError());
+    DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));   // inv must outlive its copy
+  }
+  if (!lowdin(c, field, Q, l, n, true, nullptr, 0)) return false;
+  // B = M Q^dag (m x l); residual R = M - B Q
+  const long long nB = (long long)m * l;
+  double* B = (double*)ws_get(c, WS_GSV_B, sizeof(double) * es * (size_t)nB);
+  double* R = (double*)ws_get(c, WS_GSV_R, sizeof(double) * es * (size_t)mn);
+  {
+    ProfScope ps(c, 2, (cx ? 8.0 : 2.0) * 2.0 * (double)m * n * l);
+    GemmDesc d = gdesc(m, l, n, n, n, l, field, 4);
+    d.transB = true; d.conjB = true;
+    d.pA = mn; d.pB = nQ; d.pC = nB;
+    gemm_f64(c, d, M, Q, B);
+    DMRG_CHECK_HIP(hipMemcpyAsync(R, M, sizeof(double) * es * mn, hipMemcpyDeviceToDevice, c->stream));
+    GemmDesc e = gdesc(m, n, l, l, n, n, field, 4);
+    e.alpha = -1.0; e.beta = 1.0;
+    e.pA = nB; e.pB = nQ; e.pC = mn;
+    gemm_f64(c, e, B, Q, R);
+    dev_sumsq(c, R, (long long)es * mn, scal);
+    dev_sumsq(c, M, (long long)es * mn, scal + 1);
+  }
+  double hs[2] = {0.0, 0.0};
+  DMRG_CHECK_HIP(hipMemcpyAsync(hs, scal, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+  DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  const double res2 = hs[0], tot2 = hs[1];
+  *resid = tot2 > 0.0 ? res2 / tot2 : -1.0;
+  if (!(tot2 > 0.0) || !std::isfinite(tot2)) return false;
+  const double rel = res2 / tot2;
+  const bool within_cutoff = rel <= 0.1 * cutoff;
+  // a split that maxdim truncates discards more than the cutoff anyway; the
+  // missed weight may then be up to a tenth of that forced weight
+  const bool may_be_forced = maxdim > 0 && l > maxdim;
+  if (!within_cutoff && !may_be_forced) return false;
+  // split of B on its right Gram (l x l); V = Vb Q
+  const int kc = std::min(kcap, l);
+  double* Vb = (double*)ws_get(c, WS_GSV_V, sizeof(double) * es * (size_t)kc * l);
+  SvdOut o = svd_gram(c, field, B, m, l, cutoff, maxdim, mind, absorb, U, Vb, kc, svals, 1, res2);
+  if (!within_cutoff && !(o.keep >= maxdim && rel <= 0.1 * (o.discarded - rel))) return false;
+  {
+    GemmDesc d = gdesc(o.keep, n, l, l, n, n, field, 4);
+    d.pA = (long long)o.keep * l; d.pB = nQ; d.pC = (long long)o.keep * n;
+    gemm_f64(c, d, Vb, Q, V);
+  }
+  out = o;
+  out.resid = *resid;
+  return true;
+}
+
+// Guided split entry: gside 0 = Q0 is k0 x n with orthonormal rows (the
+// factor right of the bond), gside 1 = Q0 is m x k0 with orthonormal columns
+// (the factor left of the bond; handled as the row-guided split of M^dag).
+// Falls back to svd_split when the guide is too large to pay off or the
+// subspace misses weight; *used reports which path ran.
+SvdOut svd_split_guided(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
+                        int absorb, const double* G, int k0, int gside, int sketch, double* U, double* V, int kcap,
+                        std::vector<double>& svals, int* used, double* resid) {
+  DMRG_REQUIRE(m > 0 && n > 0, "svd: empty matrix");
+  DMRG_REQUIRE(absorb >= 0 && absorb <= 2, "svd: absorb must be 0, 1 or 2");
+  DMRG_REQUIRE(gside == 0 || gside == 1, "svd: guide side must be 0 (rows) or 1 (columns)");
+  const int mind = std::max(1, mindim);
+  *used = 0;
+  *resid = -1.0;
+  const int len = gside == 0 ? n : m;   // guide vector length
+  const bool worth = G && k0 > 0 && sketch > 0 && (cutoff > 0.0 || maxdim > 0) && std::min(m, n) >= 16 &&
+                     (double)(k0 + sketch) <= 0.6 * std::min(m, n) && k0 < len;
+  if (worth) {
+    SvdOut o;
+    bool ok;
+    if (gside == 0) {
+      ok = svd_rowguided(c, field, M, m, n, G, k0, sketch, cutoff, maxdim, mind, absorb, U, V, kcap, svals, o,
+                         resid);
+    } else {
+      const bool cx = field == CPLX;
+      const size_t es = cx ? 2 : 1;
+      const long long mn = (long long)m * n, mk = (long long)m * k0;
+      double* Mt = (double*)ws_get(c, WS_GSV_T, sizeof(double) * es * mn);
+      double* Gt = (double*)ws_get(c, WS_GSV_T2, sizeof(double) * es * mk);
+      if (cx) {
+        conj_transpose_k<true><<<g1(mn), 256, 0, c->stream>>>(M, m, n, mn, Mt, mn, nullptr, 0);
+        conj_transpose_k<true><<<g1(mk), 256, 0, c->stream>>>(G, m, k0, mk, Gt, mk, nullptr, 0);
+      } else {
+        conj_transpose_k<false><<<g1(mn), 256, 0, c->stream>>>(M, m, n, 0, Mt, 0, nullptr, 0);
+        conj_transpose_k<false><<<g1(mk), 256, 0, c->stream>>>(G, m, k0, 0, Gt, 0, nullptr, 0);
+      }
+      DMRG_CHECK_HIP(hipGetLastError());
+      const int kc = std::min(kcap, std::min(m, n));
+      double* Ut = (double*)ws_get(c, WS_GSV_OU, sizeof(double) * es * (size_t)n * kc);
+      double* Vt = (double*)ws_get(c, WS_GSV_OV, sizeof(double) * es * (size_t)kc * m);
+      const int absorb_t = absorb == 0 ? 1 : (absorb == 1 ? 0 : 2);
+      ok = svd_rowguided(c, field, Mt, n, m, Gt, k0, sketch, cutoff, maxdim, mind, absorb_t, Ut, Vt, kc, svals, o,
+                         resid);
+      if (ok) {
+        // M^dag = Ut S Vt  =>  M = Vt^dag S Ut^dag
+        const int k = o.keep;
+        const long long pU = (long long)m * k, pV = (long long)k * n;
+        if (cx) {
+          conj_transpose_k<true><<<g1(pU), 256, 0, c->stream>>>(Vt, k, m, (long long)k * m, U, pU, nullptr, 0);
+          conj_transpose_k<true><<<g1(pV), 256, 0, c->stream>>>(Ut, n, k, (long long)n * k, V, pV, nullptr, 0);
+        } else {
+          conj_transpose_k<false><<<g1(pU), 256, 0, c->stream>>>(Vt, k, m, 0, U, 0, nullptr, 0);
+          conj_transpose_k<false><<<g1(pV), 256, 0, c->stream>>>(Ut, n, k, 0, V, 0, nullptr, 0);
+        }
+        DMRG_CHECK_HIP(hipGetLastError());
+      }
+    }
+    if (ok) {
+      DMRG_REQUIRE(o.keep <= kcap, "svd: keep exceeds output capacity");
+      *used = 1;
+      return o;
+    }
+  }
+  return svd_split(c, field, M, m, n, cutoff, maxdim, mindim, absorb, U, V, kcap, svals);
 }
 
 SvdOut svd_split(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
@@ -523,6 +828,26 @@ int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cut
   if (discarded) *discarded = o.discarded;
   if (svals_host)
     for (size_t i = 0; i < sv.size(); ++i) svals_host[i] = sv[i];
+  return 0;
+}
+
+int svd_split_guided_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim,
+                           int mindim, int absorb, const double* G, int k0, int gside, int sketch, double* U,
+                           double* V, int kcap, int* keep, double* discarded, double* svals_host, int* used,
+                           double* resid) {
+  std::vector<double> sv;
+  int u = 0;
+  double rs = -1.0;
+  SvdOut o = svd_split_guided(c, field, M, m, n, cutoff, maxdim, mindim, absorb, G, k0, gside, sketch, U, V, kcap,
+                              sv, &u, &rs);
+  if (resid) *resid = rs;
+  if (keep) *keep = o.keep;
+  if (discarded) *discarded = o.discarded;
+  if (used) *used = u;
+  if (svals_host) {
+    const size_t full = (size_t)std::min(m, n);
+    for (size_t i = 0; i < full; ++i) svals_host[i] = i < sv.size() ? sv[i] : 0.0;
+  }
   return 0;
 }
 

@@ -17,6 +17,8 @@ from . import _lib
 from ._lib import EnvDesc, HeffDesc, check
 
 PREC_NAMES = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+# rows of the random sketch that enlarges a guide subspace in svd_split
+SVD_SKETCH = 128
 
 
 def _ptr(t):
@@ -42,6 +44,14 @@ class Engine:
               "dmrg_ctx_create")
         self.ctx = ctx
         self._keep = []
+        self.last_svd_guided = 0
+        self.last_svd_resid = -1.0
+        # two-site splits in sweeps use the neighbouring isometry as a guide
+        # subspace (dmrg_svd_split_guided) when True.  Off by default: the
+        # full split is the reference's rule; the guided one pays where
+        # maxdim limits the bonds or the cutoff is loose.
+        self.svd_guided = False
+        self.svd_sketch = SVD_SKETCH
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -159,9 +169,15 @@ class Engine:
         e.ld_in, e.ld_out = int(env_in.shape[-1]), int(env_out.shape[-1])
         check(self.lib.dmrg_env_update(self.ctx, ctypes.byref(e)), "dmrg_env_update")
 
-    def svd_split(self, field, M, m, n, cutoff, maxdim, mindim, absorb):
+    def svd_split(self, field, M, m, n, cutoff, maxdim, mindim, absorb, guide=None, guide_side=0,
+                  sketch=None):
         """M: float64 device [np, m, n].  Returns (U [np,m,keep], V [np,keep,n],
-        keep, discarded, singular values (numpy, descending))."""
+        keep, discarded, singular values (numpy, descending)).
+
+        guide: an isometry whose span should hold most of M -- guide_side 0:
+        [np, k0, n] with orthonormal rows, 1: [np, m, k0] with orthonormal
+        columns -- for the subspace split of dmrg_svd_split_guided;
+        self.last_svd_guided tells whether that path produced the result."""
         npl = 2 if field == 1 else 1
         kcap = min(m, n) if not maxdim else min(m, n, maxdim)
         kcap = max(kcap, max(1, mindim))
@@ -170,9 +186,27 @@ class Engine:
         keep = ctypes.c_int()
         disc = ctypes.c_double()
         sv = np.zeros(min(m, n))
-        check(self.lib.dmrg_svd_split(self.ctx, field, _ptr(M), m, n, cutoff, int(maxdim or 0), mindim, absorb,
-                                      _ptr(Ub), _ptr(Vb), kcap, ctypes.byref(keep), ctypes.byref(disc),
-                                      sv.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "dmrg_svd_split")
+        svp = sv.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        if guide is None:
+            check(self.lib.dmrg_svd_split(self.ctx, field, _ptr(M), m, n, cutoff, int(maxdim or 0), mindim, absorb,
+                                          _ptr(Ub), _ptr(Vb), kcap, ctypes.byref(keep), ctypes.byref(disc), svp),
+                  "dmrg_svd_split")
+            self.last_svd_guided = 0
+            self.last_svd_resid = -1.0
+        else:
+            k0 = guide.numel() // (npl * (n if guide_side == 0 else m))
+            if guide.dtype != self.torch.float64 or not guide.is_contiguous() or \
+                    guide.numel() != npl * k0 * (n if guide_side == 0 else m):
+                raise ValueError("svd_split: guide must be a contiguous float64 [np, k0, n] or [np, m, k0] tensor")
+            used = ctypes.c_int()
+            resid = ctypes.c_double()
+            sketch = self.svd_sketch if sketch is None else sketch
+            check(self.lib.dmrg_svd_split_guided(self.ctx, field, _ptr(M), m, n, cutoff, int(maxdim or 0), mindim,
+                                                 absorb, _ptr(guide), k0, guide_side, sketch, _ptr(Ub), _ptr(Vb),
+                                                 kcap, ctypes.byref(keep), ctypes.byref(disc), svp,
+                                                 ctypes.byref(used), ctypes.byref(resid)), "dmrg_svd_split_guided")
+            self.last_svd_guided = used.value
+            self.last_svd_resid = resid.value
         k = keep.value
         U = Ub[:npl * m * k].view(npl, m, k).clone()
         V = Vb[:npl * k * n].view(npl, k, n).clone()

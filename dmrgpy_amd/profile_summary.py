@@ -3,6 +3,7 @@
     python -m dmrgpy_amd.profile_summary --applies K [--trace T.csv]
         [--fetch F.csv --write W.csv --cal-fetch CF.csv --cal-write CW.csv
          --cal-bytes B]
+    python -m dmrgpy_amd.profile_summary --stats KERNEL_STATS.csv [--top 20]
 
 An H_eff apply in reduced precision is five consecutive dispatches on the
 engine's stream: FP64->FP32 conversion of theta, the L-GEMM (kernel tag 1),
@@ -25,7 +26,7 @@ ROLES = ("to_fp32", "L-GEMM", "small-op", "R-GEMM", "to_fp64")
 
 
 def _role(name):
-    if name.startswith("void dmrg::convert_k"):
+    if name.startswith("void dmrg::convert_k") or name.startswith("void dmrg::split_transpose_k<double>"):
         return "to_fp32"
     if name.startswith("void dmrg::convert_back_k"):
         return "to_fp64"
@@ -87,9 +88,66 @@ def calibration(path, counter, nbytes):
     return nbytes / (sum(vals) / len(vals))
 
 
+CATEGORIES = (
+    ("eigensolver", "rocSOLVER/rocBLAS eigensolver (syevd/heevd) and the library GEMMs it calls"),
+    ("heff", "H_eff GEMMs (L- and R-step)"),
+    ("smallop", "small operator (MPO W) passes"),
+    ("svd", "SVD GEMMs (Gram, factors, Loewdin, guided subspace)"),
+    ("env", "environment-update GEMMs"),
+    ("grad", "gradient GEMMs"),
+    ("krylov", "Krylov vector kernels"),
+    ("convert", "precision conversions and bf16 splits"),
+    ("copy", "runtime copies and fills"),
+    ("other", "other"),
+)
+
+
+def _category(name):
+    if "rocsolver" in name or "rocblas" in name or name.startswith("Cijk_"):
+        return "eigensolver"
+    if name.startswith("void dmrg::gemm"):
+        tag = name.split(">(")[0].rsplit(",", 1)[-1].strip()
+        return {"1": "heff", "2": "heff", "3": "env", "4": "svd", "5": "grad"}.get(tag, "other")
+    if name.startswith("void dmrg::smallop"):
+        return "smallop"
+    if any(k in name for k in ("dot_block", "sub_block", "reduce_rows", "combine_k", "scale_copy", "axpy")):
+        return "krylov"
+    if any(k in name for k in ("convert", "split_rows", "split_transpose")):
+        return "convert"
+    if name.startswith("__amd_rocclr"):
+        return "copy"
+    return "other"
+
+
+def kernel_stats(path, top):
+    """Markdown tables from a rocprofv3 --stats kernel_stats.csv."""
+    rows = list(csv.DictReader(open(path)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    cat = {}
+    for r in rows:
+        k = _category(r["Name"])
+        cat[k] = cat.get(k, 0.0) + float(r["TotalDurationNs"])
+    out = ["Total kernel time {:.2f} s, {} distinct kernels.".format(tot / 1e9, len(rows)), "",
+           "| share | kernel time (s) | category |", "|---:|---:|---|"]
+    for k, desc in sorted(CATEGORIES, key=lambda kd: -cat.get(kd[0], 0.0)):
+        if k in cat:
+            out.append("| {:.1f}% | {:.3f} | {} |".format(100 * cat[k] / tot, cat[k] / 1e9, desc))
+    out += ["", "Top {} kernels:".format(top), "", "| share | total (ms) | calls | avg (us) | kernel |",
+            "|---:|---:|---:|---:|---|"]
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    for r in rows[:top]:
+        t = float(r["TotalDurationNs"])
+        out.append("| {:.2f}% | {:.1f} | {} | {:.1f} | `{}` |".format(
+            100 * t / tot, t / 1e6, r["Calls"], float(r["AverageNs"]) / 1e3, r["Name"][:110]))
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--applies", type=int, required=True)
+    ap.add_argument("--stats", help="kernel_stats.csv of a --kernel-trace --stats run: print the category and "
+                                    "top-kernel tables and exit")
+    ap.add_argument("--top", type=int, default=20)
+    ap.add_argument("--applies", type=int, default=0)
     ap.add_argument("--trace")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
@@ -97,6 +155,11 @@ def main():
     ap.add_argument("--cal-write")
     ap.add_argument("--cal-bytes", type=float, default=float(1 << 30))
     a = ap.parse_args()
+    if a.stats:
+        print(kernel_stats(a.stats, a.top))
+        return
+    if a.applies <= 0:
+        raise SystemExit("--applies K is required without --stats")
     res = {}
     if a.trace:
         ap_ = group_applies(read_dispatches(a.trace))[-a.applies:]

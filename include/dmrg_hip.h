@@ -126,6 +126,29 @@ int dmrg_svd_split(dmrg_ctx* ctx, int field, const double* M, int m, int n, doub
                    int mindim, int absorb, double* U, double* V, int kcap, int* keep,
                    double* discarded, double* svals_host);
 
+/* dmrg_svd_split with a guide: G holds an isometry whose span should contain
+ * most of M -- gside 0: k0 x n with orthonormal rows (in a right-moving
+ * sweep, the tensor right of the bond), gside 1: m x k0 with orthonormal
+ * columns (left-moving: the tensor left of the bond).  The split is then
+ * computed in span(G) enlarged by a `sketch`-column random sketch of the
+ * rest (an eigenproblem of order k0 + sketch instead of min(m, n)), and
+ * accepted only when the weight of M outside that subspace is at most
+ * 0.1 * cutoff * |M|^2, or, where maxdim forces the truncation, at most a
+ * tenth of the weight maxdim discards; that weight counts as discarded in
+ * the keep rule of _truncate (src/dmrgpy/pyitensor/svd.py:32-68).
+ * Otherwise, or when k0 + sketch > 0.6 * min(m, n), it runs dmrg_svd_split.
+ * *used = 1 when the guided path produced the result; *resid (may be NULL)
+ * receives the measured weight outside the subspace over |M|^2 (-1 when the
+ * guided path was not attempted).  The reference always
+ * runs the full split (svd.py:108-151, called from dmrg.py:336-352); this
+ * entry has no counterpart there and is what makes the truncation cost
+ * scale with the kept dimension.  svals_host (length min(m, n)) receives
+ * the k0 + sketch leading values, zeros after. */
+int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int n, double cutoff, int maxdim,
+                          int mindim, int absorb, const double* G, int k0, int gside, int sketch, double* U,
+                          double* V, int kcap, int* keep, double* discarded, double* svals_host, int* used,
+                          double* resid);
+
 /* Gradient of <bra|H|ket> with respect to one MPO site tensor W[wl][wr][s'][s]:
  * G = sum L[wl][a'][a] conj(bra[a'][s'][b']) ket[a][s][b] R[wr][b][b'], written
  * to host memory as G[np][wl][wr][d][d] (FP64).  L [wl][Dlb][Dl] and
