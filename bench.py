@@ -13,9 +13,11 @@ N=64, t=1, U=4 (particle-hole symmetric form), open boundaries, w=6, D ramped
 A step is one full DMRG sweep (left-to-right and back, 2(N-1) bond updates:
 Lanczos ground state of H_eff, FP64 SVD truncation, environment update).
 Setup (untimed): seeded random MPS, the D ramp and W warm-up sweeps at D.
-Timed: K sweeps in the chosen precision; then K sweeps in FP64 from the same
-starting state for the comparison (|dE|/N between the two final FP64
-Rayleigh quotients).  H_eff throughput comes from HIP events recorded
+Timed: K sweeps in the chosen precision; then, from the same starting
+state, K sweeps with rocSOLVER dsyevd in place of the library's own
+tridiagonalisation as the Gram eigensolver (the A/B of that kernel), and K
+sweeps in FP64 for the comparison (|dE|/N between the final FP64 Rayleigh
+quotients).  H_eff throughput comes from HIP events recorded
 around every H_eff apply on the engine's stream.  The CPU baseline times the
 oracle (NumPy complex128, the reference algorithm) on a bounded sample of
 the same workload: one H_eff matvec, one truncated SVD and one environment
@@ -356,6 +358,8 @@ def main():
     ap.add_argument("--svd", default="full", choices=["guided", "full"],
                     help="two-site splits: the full split (the reference's rule), or the subspace split guided "
                          "by the neighbouring isometry (falls back to the full split when it misses weight)")
+    ap.add_argument("--no-syevd-check", action="store_true",
+                    help="skip the timed sweep with rocSOLVER dsyevd as the Gram eigensolver from the same state")
     ap.add_argument("--no-full-svd-check", action="store_true",
                     help="skip the timed sweep with the full split from the same state (guided runs only)")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -413,6 +417,12 @@ def main():
     start = psi.copy()
     res = run_sweeps(ch, psi, args.steps, args.precision, D)
     log("timed {}: {}".format(args.precision, res))
+    ressy = None
+    if not args.no_syevd_check:
+        eng.set_eigensolver(0)
+        ressy = run_sweeps(ch, start.copy(), args.steps, args.precision, D)
+        eng.set_eigensolver(1)
+        log("timed {} with rocSOLVER dsyevd: {}".format(args.precision, ressy))
     resfull = None
     if args.svd == "guided" and not args.no_full_svd_check:
         eng.svd_guided = False
@@ -473,6 +483,12 @@ def main():
                                "averaged over the sweep's bonds{}, {:.3f} ms average".format(
                                    flops_per / 1e9, " (x3 BF16 products)" if mfma_mult > 1 else "", avg_ms)},
     }
+    if ressy is not None:
+        out["rocsolver_syevd"] = {"sweep_s": ressy["sweep_s"], "energy": ressy["energy"],
+                                  "svd_s_per_sweep": ressy["svd_ms"] / 1e3 / args.steps,
+                                  "max_bond": ressy["max_bond"],
+                                  "dE_per_site_vs_own_eigensolver": abs(res["energy"] - ressy["energy"]) / n}
+        out["speedup_own_eigensolver_vs_syevd"] = ressy["sweep_s"] / res["sweep_s"]
     if resfull is not None:
         out["full_svd"] = {"sweep_s": resfull["sweep_s"], "energy": resfull["energy"],
                            "time_split_s_per_sweep": {"heff": resfull["heff_ms"] / 1e3 / args.steps,

@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdmrg_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
-SOURCES = ["gemm.hip", "contract.hip", "krylov.hip", "svd.hip", "capi.hip", "grad.hip"]
+SOURCES = ["gemm.hip", "contract.hip", "krylov.hip", "svd.hip", "sytrd.hip", "capi.hip", "grad.hip"]
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result"]

@@ -19,6 +19,7 @@ void krylov_expm(Ctx* c, const HeffOp& h, double* x, std::complex<double> coeff,
 double rayleigh(Ctx* c, const HeffOp& h, const double* x);
 int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
                     int absorb, double* U, double* V, int kcap, int* keep, double* discarded, double* svals_host);
+int sym_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z);
 int svd_split_guided_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim,
                            int mindim, int absorb, const double* G, int k0, int gside, int sketch, double* U,
                            double* V, int kcap, int* keep, double* discarded, double* svals_host, int* used,
@@ -216,6 +217,22 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
   DMRG_REQUIRE(k0 >= 0 && sketch >= 0, "svd_split_guided: negative guide size");
   dmrg::svd_split_guided_entry(&ctx->c, field, M, m, n, cutoff, maxdim, mindim, absorb, G, k0, gside, sketch, U, V,
                                kcap, keep, discarded, svals_host, used, resid);
+  API_END
+}
+
+int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx, "set_eigensolver: null context");
+  DMRG_REQUIRE(mode == 0 || mode == 1, "set_eigensolver: mode must be 0 (rocSOLVER syevd) or 1 (own tridiagonalisation)");
+  ctx->c.eig_mode = mode;
+  API_END
+}
+
+int dmrg_sym_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && A && w && Z, "sym_eig: null argument");
+  DMRG_REQUIRE(mode == 0 || mode == 1, "sym_eig: mode must be 0 or 1");
+  dmrg::sym_eig_entry(&ctx->c, mode, A, n, w, Z);
   API_END
 }
 

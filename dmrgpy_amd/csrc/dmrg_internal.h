@@ -61,6 +61,8 @@ void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, 
                  int inner, long long sx_in, long long sx_out, long long pin, long long pout);
 void gemm_prec(Ctx* c, int prec, const GemmDesc& g, const void* A, const void* B, void* C);
 bool smallop_sparse_ok(const SmallOp& s);
+// dsytrd (uplo 'L') of a symmetric n x n matrix in place: d[n], e[n-1], tau[n-1]
+void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau);
 GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag);
 
 double heff_flops(const dmrg_heff_desc& d);

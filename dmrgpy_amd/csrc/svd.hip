@@ -413,6 +413,7 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
   double* amax_dev = ev + 2 * (size_t)g + 8;
   rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
   double* Gsolve = nullptr;
+  double *trd_Z = nullptr, *trd_tau = nullptr, *trd_refl = nullptr;
   {
     ProfScope ps(c, 2, (cx ? 8.0 : 2.0) * (double)g * g * (left ? n : m));
     double* Gplanar = (double*)ws_get(c, WS_SVD_TMP2, sizeof(double) * es * (size_t)g * g);
@@ -438,6 +439,15 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
                                 g, ev, ework, info),
                "zheevd");
       Gsolve = Gi;
+    } else if (c->eig_mode == 1 && g >= 32) {
+      // tridiagonalise here, eigenpairs of the tridiagonal by rocSOLVER's
+      // divide and conquer; the back-transformation waits for keep
+      trd_Z = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * ((size_t)g * g + g + 16));
+      trd_tau = trd_Z + (size_t)g * g;
+      sytrd_lower(c, Gplanar, g, ev, ework, trd_tau);
+      check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, g, ev, ework, trd_Z, g, info), "dstedc");
+      Gsolve = trd_Z;
+      trd_refl = Gplanar;
     } else {
       check_rb(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, g, Gplanar, g, ev, ework, info),
                "dsyevd");
@@ -461,6 +471,14 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
   if (!(svals[0] > 0.0 && svals[keep - 1] >= 1e-7 * svals[0]))
     return svd_full(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals);
   DMRG_REQUIRE(keep <= kcap, "svd: keep exceeds output capacity");
+  if (trd_refl) {
+    // eigenvectors of G for the kept (largest) values only: Q times the
+    // last keep eigenvector columns of the tridiagonal
+    ProfScope ps(c, 2, 2.0 * (double)g * g * keep);
+    check_rb(rocsolver_dormtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, g, keep,
+                              trd_refl, g, trd_tau, trd_Z + (size_t)(g - keep) * g, g),
+             "dormtr");
+  }
   // device copy of the kept singular values
   double* sdev = ev;  // reuse: ev no longer needed
   {
@@ -828,6 +846,33 @@ int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cut
   if (discarded) *discarded = o.discarded;
   if (svals_host)
     for (size_t i = 0; i < sv.size(); ++i) svals_host[i] = sv[i];
+  return 0;
+}
+
+// All eigenpairs of a real symmetric matrix (row-major = column-major):
+// w ascending, Z[k][:] the eigenvector of w[k] (rows of the row-major Z).
+// mode 1: own tridiagonalisation + stedc + ormtr, mode 0: syevd.  A is
+// overwritten.
+int sym_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
+  DMRG_REQUIRE(n >= 2, "sym_eig: order must be at least 2");
+  double* ework = (double*)ws_get(c, WS_SVD_EV, sizeof(double) * ((size_t)2 * n + 16));
+  double* tauv = ework + n;
+  rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
+  if (mode == 1) {
+    sytrd_lower(c, A, n, w, ework, tauv);
+    check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, n, w, ework, Z, n, info), "dstedc");
+    check_rb(rocsolver_dormtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, A, n,
+                              tauv, Z, n),
+             "dormtr");
+  } else {
+    check_rb(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, n, w, ework, info),
+             "dsyevd");
+    DMRG_CHECK_HIP(hipMemcpyAsync(Z, A, sizeof(double) * n * n, hipMemcpyDeviceToDevice, c->stream));
+  }
+  int hinfo = 0;
+  DMRG_CHECK_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  DMRG_REQUIRE(hinfo == 0, "sym_eig: eigensolver did not converge");
   return 0;
 }
 

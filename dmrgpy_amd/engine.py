@@ -212,6 +212,21 @@ class Engine:
         V = Vb[:npl * k * n].view(npl, k, n).clone()
         return U, V, k, disc.value, sv
 
+    def set_eigensolver(self, mode):
+        """1: own tridiagonalisation + rocSOLVER stedc/ormtr (default);
+        0: rocSOLVER syevd -- for the Gram-route splits."""
+        check(self.lib.dmrg_set_eigensolver(self.ctx, int(mode)), "dmrg_set_eigensolver")
+
+    def sym_eig(self, A, mode=1):
+        """Eigenpairs of a symmetric float64 device matrix [n, n] (copied):
+        (w ascending [n], Z [n, n] with row k the eigenvector of w[k])."""
+        n = A.shape[-1]
+        Ac = A.reshape(n, n).clone()
+        w = self.empty((n,))
+        Z = self.empty((n, n))
+        check(self.lib.dmrg_sym_eig(self.ctx, int(mode), _ptr(Ac), n, _ptr(w), _ptr(Z)), "dmrg_sym_eig")
+        return w, Z
+
     def mpo_grad(self, field, prec, L, bra, ket, R, wl, wr):
         """dE/dW of one site: host [np, wl, wr, d, d] float64."""
         _, Dl, d, Dr = ket.shape
