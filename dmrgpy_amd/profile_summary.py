@@ -30,7 +30,7 @@ def _role(name):
         return "to_fp32"
     if name.startswith("void dmrg::convert_back_k"):
         return "to_fp64"
-    if name.startswith("void dmrg::smallop"):
+    if "smallop" in name:   # the split-output variant's name reaches the CSV mangled
         return "small-op"
     if name.startswith("void dmrg::gemm"):
         tag = name.split(">(")[0].rsplit(",", 1)[-1].strip()
@@ -89,7 +89,9 @@ def calibration(path, counter, nbytes):
 
 
 CATEGORIES = (
-    ("eigensolver", "rocSOLVER/rocBLAS eigensolver (syevd/heevd) and the library GEMMs it calls"),
+    ("trd", "own Householder tridiagonalisation (trd_col_k, trd_symv_k, trd_fin_k; its panel GEMMs count as SVD GEMMs)"),
+    ("eigensolver", "rocSOLVER/rocBLAS: tridiagonal divide and conquer (stedc), back-transformation (ormtr), "
+                    "syevd/heevd where used, and the library GEMMs they call"),
     ("heff", "H_eff GEMMs (L- and R-step)"),
     ("smallop", "small operator (MPO W) passes"),
     ("svd", "SVD GEMMs (Gram, factors, Loewdin, guided subspace)"),
@@ -103,12 +105,14 @@ CATEGORIES = (
 
 
 def _category(name):
+    if "trd_" in name:
+        return "trd"
     if "rocsolver" in name or "rocblas" in name or name.startswith("Cijk_"):
         return "eigensolver"
     if name.startswith("void dmrg::gemm"):
         tag = name.split(">(")[0].rsplit(",", 1)[-1].strip()
         return {"1": "heff", "2": "heff", "3": "env", "4": "svd", "5": "grad"}.get(tag, "other")
-    if name.startswith("void dmrg::smallop"):
+    if "smallop" in name:
         return "smallop"
     if any(k in name for k in ("dot_block", "sub_block", "reduce_rows", "combine_k", "scale_copy", "axpy")):
         return "krylov"
