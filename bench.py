@@ -123,6 +123,8 @@ def run_sweeps(ch, psi, K, prec_name, maxdim):
     return {"sweep_s": dt / K, "energy": e, "heff_ms": tms[0], "heff_flops": flops[0], "heff_calls": calls[0],
             "heff_bytes": nbytes[0],
             "env_ms": tms[1], "svd_ms": tms[2], "svd_calls": calls[2],
+            "trd_ms": tms[4], "trd_calls": calls[4], "trd_bytes": nbytes[4],
+            "vec_ms": tms[3], "vec_calls": calls[3], "vec_bytes": nbytes[3],
             "iters_per_bond": sum(stats.lanczos_iters) / max(1, len(stats.lanczos_iters)),
             "svd_guided_fraction": sum(stats.svd_guided) / max(1, len(stats.svd_guided)),
             "max_bond": max(psi.bond_dims())}
@@ -483,6 +485,24 @@ def main():
                                "averaged over the sweep's bonds{}, {:.3f} ms average".format(
                                    flops_per / 1e9, " (x3 BF16 products)" if mfma_mult > 1 else "", avg_ms)},
     }
+    if res["trd_calls"] > 0 and res["trd_ms"] > 0:
+        # the Gram eigensolver's tridiagonalisation, the largest cost of a sweep:
+        # algorithmic bytes = the lower triangle of the trailing matrix once per column
+        gbs = res["trd_bytes"] / (res["trd_ms"] * 1e-3) / 1e9
+        out["roofline_tridiagonalisation"] = {
+            "bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0, "traffic": None,
+            "algorithmic_bytes": res["trd_bytes"] / res["trd_calls"],
+            "kernel": "own Householder tridiagonalisation of the Gram matrices (trd_col_k + trd_symv_k per column, "
+                      "panel GEMMs), {} reductions per sweep, {:.2f} ms average, {:.2f} s per sweep".format(
+                          int(res["trd_calls"] / args.steps), res["trd_ms"] / res["trd_calls"],
+                          res["trd_ms"] / 1e3 / args.steps)}
+    if res["vec_calls"] > 0 and res["vec_ms"] > 0:
+        # Lanczos vector passes (alpha/beta dots, re-orthogonalisation dot and
+        # subtract, norms): bytes = the vectors each pass reads and writes once
+        vgbs = res["vec_bytes"] / (res["vec_ms"] * 1e-3) / 1e9
+        out["krylov_vectors"] = {"bound": "hbm", "achieved": vgbs, "peak": 8000.0, "unit": "GB/s",
+                                 "frac": vgbs / 8000.0, "passes_per_sweep": int(res["vec_calls"] / args.steps),
+                                 "s_per_sweep": res["vec_ms"] / 1e3 / args.steps}
     if ressy is not None:
         out["rocsolver_syevd"] = {"sweep_s": ressy["sweep_s"], "energy": ressy["energy"],
                                   "svd_s_per_sweep": ressy["svd_ms"] / 1e3 / args.steps,

@@ -276,7 +276,7 @@ int dmrg_prof_enable(dmrg_ctx* ctx, int on) {
   API_BEGIN
   dmrg::Ctx& c = ctx->c;
   dmrg::prof_drain(&c, true);
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < 5; ++i) {
     c.prof_time_ms[i] = 0;
     c.prof_flops[i] = 0;
     c.prof_bytes[i] = 0;
@@ -290,7 +290,7 @@ int dmrg_prof_read(dmrg_ctx* ctx, double* time_ms, double* flops, long long* cal
   API_BEGIN
   dmrg::Ctx& c = ctx->c;
   dmrg::prof_drain(&c, true);
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < 5; ++i) {
     if (time_ms) time_ms[i] = c.prof_time_ms[i];
     if (flops) flops[i] = c.prof_flops[i];
     if (calls) calls[i] = c.prof_calls[i];
@@ -302,7 +302,7 @@ int dmrg_prof_read_bytes(dmrg_ctx* ctx, double* bytes) {
   API_BEGIN
   dmrg::Ctx& c = ctx->c;
   dmrg::prof_drain(&c, true);
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 5; ++i)
     if (bytes) bytes[i] = c.prof_bytes[i];
   API_END
 }

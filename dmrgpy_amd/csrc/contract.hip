@@ -652,10 +652,14 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   {
     const double np_ = d.field == CPLX ? 2.0 : 1.0, es = (double)prec_size(d.prec);
     const double n = (double)h.n, t1 = (double)wl * Dl * h.P * Dr, t2 = (double)Dl * h.P * wr * Dr;
+    // FP32 real: the R-GEMM writes y in FP64 itself (no conversion pass)
+    const bool direct64 = d.prec == P32 && d.field == REAL;
     double b = np_ * es * ((double)wl * Dl * Dl + n + t1     // L-GEMM: L, x in, T1 out
                            + t1 + t2                          // small operator: T1 in, T2 out
-                           + t2 + (double)wr * Dr * Dr + n);  // R-GEMM: T2, R in, y out
-    if (d.prec != P64) b += np_ * n * (8.0 + es) * 2.0;       // x to and y from FP64
+                           + t2 + (double)wr * Dr * Dr);      // R-GEMM: T2, R in
+    b += np_ * n * (direct64 ? 8.0 : es);                     // R-GEMM: y out
+    if (d.prec != P64) b += np_ * n * (8.0 + es);             // x to the contraction precision
+    if (d.prec != P64 && !direct64) b += np_ * n * (es + 8.0);  // y back to FP64
     h.bytes_per_apply = b;
   }
   if (d.prec == PB3 && d.field == REAL && smallop_sparse_ok(h.sop)) {
@@ -750,7 +754,9 @@ void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
     to_prec_pitched(c, d.prec, x, xb, Dl, P * Dr, ldx, np);
     xin = xb;
   }
-  if (d.prec != P64) yout = ws_get(c, WS_YOUT, n * np * es);
+  // FP32 real: the R-GEMM writes y in FP64 directly (no conversion pass)
+  const bool direct64 = d.prec == P32 && d.field == REAL;
+  if (d.prec != P64 && !direct64) yout = ws_get(c, WS_YOUT, n * np * es);
   void* T1 = ws_get(c, WS_T1, nT * np * es);
   void* T2 = ws_get(c, WS_T2, nT2 * np * es);
   {  // T1[(wl a')][(p b)] = L[(wl a')][a] * x[a][(p b)]
@@ -762,9 +768,10 @@ void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y) {
   {  // y[(a' p')][b'] = T2[(a' p')][(wr b)] * R[(wr b)][b']
     GemmDesc g = gdesc(Dl * P, Dr, wr * Dr, ldT2, ldR, Dr, d.field, 2);
     g.pA = nT2; g.pB = (long long)wr * Dr * ldR; g.pC = n;
-    gemm_prec(c, d.prec, g, T2, d.R, yout);
+    if (direct64) gemm_f32_c64(c, g, (const float*)T2, (const float*)d.R, y);
+    else gemm_prec(c, d.prec, g, T2, d.R, yout);
   }
-  if (d.prec != P64) from_prec(c, d.prec, yout, y, n * np);
+  if (d.prec != P64 && !direct64) from_prec(c, d.prec, yout, y, n * np);
   apply_projectors(c, h, x, y);
   c->matvecs++;
 }

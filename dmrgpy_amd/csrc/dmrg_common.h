@@ -92,14 +92,15 @@ struct Ctx {
   rocblas_handle blas = nullptr;
   DevBuf ws[WS_NSLOTS];
   double* host_scalars = nullptr;  // pinned, for scalar read-back
-  // profiling of H_eff applies (kind 0) and environment updates (kind 1)
+  // profiling: H_eff applies (kind 0), environment updates (1), SVDs (2),
+  // Krylov vector work (3), tridiagonalisations (4)
   bool prof_on = false;
   std::vector<ProfEntry> prof;
   std::vector<hipEvent_t> event_pool;
-  double prof_time_ms[4] = {0, 0, 0, 0};
-  double prof_flops[4] = {0, 0, 0, 0};
-  double prof_bytes[4] = {0, 0, 0, 0};   // algorithmic HBM bytes
-  long long prof_calls[4] = {0, 0, 0, 0};
+  double prof_time_ms[5] = {0, 0, 0, 0, 0};
+  double prof_flops[5] = {0, 0, 0, 0, 0};
+  double prof_bytes[5] = {0, 0, 0, 0, 0};   // algorithmic HBM bytes
+  long long prof_calls[5] = {0, 0, 0, 0, 0};
   long long matvecs = 0;
   // dense symmetric eigensolver of the real Gram route: 1 = own two-launch
   // tridiagonalisation + rocSOLVER stedc / ormtr (eigenvectors back-
@@ -130,6 +131,8 @@ struct GemmDesc {
 };
 void gemm_f64(Ctx* c, const GemmDesc& g, const double* A, const double* B, double* C);
 void gemm_f32(Ctx* c, const GemmDesc& g, const float* A, const float* B, float* C);
+// real FP32 operands, FP64 result (beta must be 0)
+void gemm_f32_c64(Ctx* c, const GemmDesc& g, const float* A, const float* B, double* C);
 // split-BF16x3: FP32 operands split on the fly into bf16 hi/lo pairs.
 void gemm_bf16x3(Ctx* c, const GemmDesc& g, const float* A, const float* B, float* C);
 // split-BF16x3 on pre-split operands: C[M][N] = A[M][K] * B[N][K]^T, A and B

@@ -40,7 +40,7 @@ struct HeffOp {
 
 // Records a pair of HIP events around a region on the context stream when
 // profiling is on; kind 0 = H_eff apply, 1 = environment update, 2 = SVD,
-// 3 = Krylov vector work.
+// 3 = Krylov vector work, 4 = tridiagonalisation.
 struct ProfScope {
   Ctx* c;
   int kind;

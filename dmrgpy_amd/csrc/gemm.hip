@@ -195,9 +195,9 @@ template <typename T> struct Vec;
 template <> struct Vec<float> { typedef flt4 type; static constexpr int W = 4; };
 template <> struct Vec<double> { typedef double dbl2 __attribute__((ext_vector_type(2))); typedef dbl2 type; static constexpr int W = 2; };
 
-template <typename T, int TM, int TN, bool TA, bool TB, int TAG>
+template <typename T, int TM, int TN, bool TA, bool TB, int TAG, typename CT = T>
 __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const T* __restrict__ B,
-                                                 T* __restrict__ C, GemmArgs g) {
+                                                 CT* __restrict__ C, GemmArgs g) {
   constexpr int BM = 32 * TM, BN = 32 * TN, BK = 16;
   constexpr int PAD = 16 / sizeof(T);     // keeps rows 16-byte aligned
   constexpr int VW = Vec<T>::W;
@@ -357,8 +357,8 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
         if (row < M && col < N) {
           const long long off = (long long)row * g.ldc + col;
           T v = alpha * acc[a][b][r];
-          if (g.beta != 0.0) v += beta * C[off];
-          C[off] = v;
+          if (g.beta != 0.0) v += beta * (T)C[off];
+          C[off] = (CT)v;
         }
       }
 }
@@ -741,15 +741,15 @@ static void launch_tt(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
   else launch_tagged<T, CX, TM, TN, true, true>(c, d, A, B, C);
 }
 
-template <typename T, int TM, int TN, bool TA, bool TB>
-static void launch_fast_tagged(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C, bool vec) {
+template <typename T, int TM, int TN, bool TA, bool TB, typename CT = T>
+static void launch_fast_tagged(Ctx* c, const GemmDesc& d, const T* A, const T* B, CT* C, bool vec) {
   GemmArgs g = make_args(d);
   g.vec = vec ? 1 : 0;
   dim3 grid((d.N + 32 * TN - 1) / (32 * TN), (d.M + 32 * TM - 1) / (32 * TM), d.batch);
   DMRG_REQUIRE(grid.y <= 65535, "gemm: M too large for one launch");
   switch (d.tag) {
 #define CASE(t) \
-  case t: gemm_fast<T, TM, TN, TA, TB, t><<<grid, 256, 0, c->stream>>>(A, B, C, g); break;
+  case t: gemm_fast<T, TM, TN, TA, TB, t, CT><<<grid, 256, 0, c->stream>>>(A, B, C, g); break;
     DMRG_GEMM_TAGS(CASE)
 #undef CASE
     default: throw Error("gemm: bad tag");
@@ -757,12 +757,12 @@ static void launch_fast_tagged(Ctx* c, const GemmDesc& d, const T* A, const T* B
   DMRG_CHECK_HIP(hipGetLastError());
 }
 
-template <typename T, int TM, int TN>
-static void launch_fast(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C, bool vec) {
-  if (!d.transA && !d.transB) launch_fast_tagged<T, TM, TN, false, false>(c, d, A, B, C, vec);
-  else if (d.transA && !d.transB) launch_fast_tagged<T, TM, TN, true, false>(c, d, A, B, C, vec);
-  else if (!d.transA && d.transB) launch_fast_tagged<T, TM, TN, false, true>(c, d, A, B, C, vec);
-  else launch_fast_tagged<T, TM, TN, true, true>(c, d, A, B, C, vec);
+template <typename T, int TM, int TN, typename CT = T>
+static void launch_fast(Ctx* c, const GemmDesc& d, const T* A, const T* B, CT* C, bool vec) {
+  if (!d.transA && !d.transB) launch_fast_tagged<T, TM, TN, false, false, CT>(c, d, A, B, C, vec);
+  else if (d.transA && !d.transB) launch_fast_tagged<T, TM, TN, true, false, CT>(c, d, A, B, C, vec);
+  else if (!d.transA && d.transB) launch_fast_tagged<T, TM, TN, false, true, CT>(c, d, A, B, C, vec);
+  else launch_fast_tagged<T, TM, TN, true, true, CT>(c, d, A, B, C, vec);
 }
 
 // 16-byte vector loads are legal when every row start is 16-byte aligned
@@ -771,6 +771,20 @@ static bool vec_ok(const GemmDesc& d, const T* A, const T* B) {
   const int vw = 16 / (int)sizeof(T);
   auto al = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
   return al(A) && al(B) && d.lda % vw == 0 && d.ldb % vw == 0 && d.sA % vw == 0 && d.sB % vw == 0;
+}
+
+// Real FP32 GEMM writing an FP64 result (the R-step of the FP32 H_eff apply
+// writes y straight into the FP64 Krylov vector, with no conversion pass).
+void gemm_f32_c64(Ctx* c, const GemmDesc& d, const float* A, const float* B, double* C) {
+  DMRG_REQUIRE(d.field == REAL && d.beta == 0.0, "gemm_f32_c64: real field and beta = 0 only");
+  check_desc(d);
+  if (d.M == 0 || d.N == 0) return;
+  const long long tiles128 = (long long)((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch;
+  const int vw = 4;
+  auto al = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  const bool vec = al(A) && al(B) && d.lda % vw == 0 && d.ldb % vw == 0 && d.sA % vw == 0 && d.sB % vw == 0;
+  if (tiles128 >= 256) launch_fast<float, 4, 4, double>(c, d, A, B, C, vec);
+  else launch_fast<float, 2, 2, double>(c, d, A, B, C, vec);
 }
 
 // Tile choice: large problems use 128x128 (real) / 64x64 (complex) tiles;

@@ -145,6 +145,8 @@ static int dot_blocks(long long n) {
 void vec_dot_block(Ctx* c, int field, const double* Q, long long ldq, int k, const double* w, long long n,
                    double* out) {
   if (k <= 0) return;
+  // bytes: the k basis vectors and w read once
+  ProfScope ps(c, 3, 0.0, 8.0 * (field == CPLX ? 2 : 1) * (double)(k + 1) * (double)n);
   const int nb = dot_blocks(n);
   const int rows = (field == CPLX ? 2 : 1) * k;
   double* part = (double*)ws_get(c, WS_KRYLOV_RED, sizeof(double) * (size_t)rows * nb);
@@ -158,6 +160,8 @@ void vec_dot_block(Ctx* c, int field, const double* Q, long long ldq, int k, con
 void vec_sub_block(Ctx* c, int field, const double* Q, long long ldq, int k, const double* coef, double* w,
                    long long n) {
   if (k <= 0) return;
+  // bytes: the k basis vectors read once, w read and written
+  ProfScope ps(c, 3, 0.0, 8.0 * (field == CPLX ? 2 : 1) * (double)(k + 2) * (double)n);
   if (field == CPLX) sub_block_k<true><<<vgrid(n), 256, 0, c->stream>>>(Q, ldq, k, coef, w, n);
   else sub_block_k<false><<<vgrid(n), 256, 0, c->stream>>>(Q, ldq, k, coef, w, n);
   DMRG_CHECK_HIP(hipGetLastError());

@@ -321,6 +321,10 @@ void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
   double* part2 = part1 + (size_t)gmax1 * TRD_PS;
   TrdState* st = reinterpret_cast<TrdState*>(part2 + gmax2 + 8);
   DMRG_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(TrdState), c->stream));
+  // algorithmic bytes: the lower triangle of the trailing matrix, once per column
+  double bytes = 0.0;
+  for (int j = 0; j < n - 1; ++j) bytes += 8.0 * (double)(n - j - 1) * (n - j) / 2.0;
+  ProfScope ps(c, 4, 0.0, bytes);
   int g2prev = 1;
   for (int j0 = 0; j0 < n - 1; j0 += TRD_NB) {
     const int nbp = std::min(TRD_NB, n - 1 - j0);

@@ -251,14 +251,16 @@ class Engine:
         check(self.lib.dmrg_prof_enable(self.ctx, int(on)), "dmrg_prof_enable")
 
     def prof_read(self):
-        t = (ctypes.c_double * 4)()
-        f = (ctypes.c_double * 4)()
-        n = (ctypes.c_longlong * 4)()
+        """(time_ms, flops, calls) per slot: H_eff, env, SVD, Krylov,
+        tridiagonalisation."""
+        t = (ctypes.c_double * 5)()
+        f = (ctypes.c_double * 5)()
+        n = (ctypes.c_longlong * 5)()
         check(self.lib.dmrg_prof_read(self.ctx, t, f, n), "dmrg_prof_read")
         return list(t), list(f), list(n)
 
     def prof_read_bytes(self):
-        b = (ctypes.c_double * 4)()
+        b = (ctypes.c_double * 5)()
         check(self.lib.dmrg_prof_read_bytes(self.ctx, b), "dmrg_prof_read_bytes")
         return list(b)
 

@@ -33,9 +33,17 @@ def _role(name):
     if "smallop" in name:   # the split-output variant's name reaches the CSV mangled
         return "small-op"
     if name.startswith("void dmrg::gemm"):
-        tag = name.split(">(")[0].rsplit(",", 1)[-1].strip()
+        tag = _gemm_tag(name)
         return {"1": "L-GEMM", "2": "R-GEMM"}.get(tag, "gemm-other")
     return "other"
+
+
+def _gemm_tag(name):
+    """The tag template argument of a dmrg GEMM kernel name: the last integer
+    argument (an output-type argument may follow it)."""
+    args = [a.strip() for a in name.split("<", 1)[1].split(">(")[0].split(",")]
+    ints = [a for a in args if a.lstrip("-").isdigit()]
+    return ints[-1] if ints else ""
 
 
 def read_dispatches(path, counter=None):
@@ -110,7 +118,7 @@ def _category(name):
     if "rocsolver" in name or "rocblas" in name or name.startswith("Cijk_"):
         return "eigensolver"
     if name.startswith("void dmrg::gemm"):
-        tag = name.split(">(")[0].rsplit(",", 1)[-1].strip()
+        tag = _gemm_tag(name)
         return {"1": "heff", "2": "heff", "3": "env", "4": "svd", "5": "grad"}.get(tag, "other")
     if "smallop" in name:
         return "smallop"
