@@ -184,8 +184,12 @@ int dmrg_gemm(dmrg_ctx* ctx, int prec, int field, int transA, int transB, int co
               const void* B, int ldb, long long strideB, long long planeB, double beta, void* C, int ldc,
               long long strideC, long long planeC, int batch);
 
-/* Profiling of H_eff applies (slot 0), environment updates (1), SVDs (2):
- * HIP events around each call on the context stream. */
+/* Profiling of H_eff applies (slot 0), environment updates (1), SVDs (2),
+ * Krylov vector work (3) and the tridiagonalisations of the Gram
+ * eigensolver (4): HIP events around each call on the context stream.  The
+ * read functions fill arrays of 5; bytes are algorithmic (H_eff: every
+ * operand read once and every result written once; tridiagonalisation: the
+ * lower triangle of the trailing matrix read once per column). */
 int dmrg_prof_enable(dmrg_ctx* ctx, int on);
 int dmrg_prof_read(dmrg_ctx* ctx, double* time_ms, double* flops, long long* calls);
 /* Algorithmic HBM bytes of the profiled calls per slot (H_eff: every operand
