@@ -92,8 +92,10 @@ __global__ __launch_bounds__(256) void trd_col_k(const double* __restrict__ A, i
   // ---- loads
   const double tprev = st->t;
   double ps = 0.0;   // this thread's share of column j-1's w^T v partials
-  if (jj > 0)
+  if (jj > 0) {
+#pragma unroll 4
     for (int b = tid; b < g2prev; b += 256) ps += part2[b];
+  }
   // panel rows into registers first (all loads in flight at once), then LDS
   constexpr int SPT = TRD_ROWS1 * TRD_NB / 256;
   double lv[SPT], lw[SPT];
@@ -195,8 +197,10 @@ __global__ __launch_bounds__(256) void trd_symv_k(double* __restrict__ A, int n,
   double ps = 0.0;
   {
     const int k = tid & 63, grp = tid >> 6;
-    if (k < nk)
+    if (k < nk) {
+#pragma unroll 4
       for (int b = grp; b < g1; b += 4) ps += part1[(size_t)b * TRD_PS + k];
+    }
   }
   double a0 = 0.0, aj = 0.0;
   if (tid == 0) {
