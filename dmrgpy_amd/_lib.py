@@ -58,6 +58,7 @@ SIGNATURES = {
                                       P_double, P_int, P_double]),
     "dmrg_set_eigensolver": (c_int, [c_void_p, c_int]),
     "dmrg_sym_eig": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    "dmrg_herm_eig": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "dmrg_mpo_grad": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
                               c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p]),
     "dmrg_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,

@@ -20,6 +20,7 @@ double rayleigh(Ctx* c, const HeffOp& h, const double* x);
 int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim, int mindim,
                     int absorb, double* U, double* V, int kcap, int* keep, double* discarded, double* svals_host);
 int sym_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z);
+int herm_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z);
 int svd_split_guided_entry(Ctx* c, int field, const double* M, int m, int n, double cutoff, int maxdim,
                            int mindim, int absorb, const double* G, int k0, int gside, int sketch, double* U,
                            double* V, int kcap, int* keep, double* discarded, double* svals_host, int* used,
@@ -233,6 +234,14 @@ int dmrg_sym_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z
   DMRG_REQUIRE(ctx && A && w && Z, "sym_eig: null argument");
   DMRG_REQUIRE(mode == 0 || mode == 1, "sym_eig: mode must be 0 or 1");
   dmrg::sym_eig_entry(&ctx->c, mode, A, n, w, Z);
+  API_END
+}
+
+int dmrg_herm_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && A && w && Z, "herm_eig: null argument");
+  DMRG_REQUIRE(mode == 0 || mode == 1, "herm_eig: mode must be 0 or 1");
+  dmrg::herm_eig_entry(&ctx->c, mode, A, n, w, Z);
   API_END
 }
 

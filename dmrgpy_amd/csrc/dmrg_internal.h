@@ -63,6 +63,10 @@ void gemm_prec(Ctx* c, int prec, const GemmDesc& g, const void* A, const void* B
 bool smallop_sparse_ok(const SmallOp& s);
 // dsytrd (uplo 'L') of a symmetric n x n matrix in place: d[n], e[n-1], tau[n-1]
 void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau);
+// zhetrd (uplo 'L') of the column-major view of a Hermitian n x n planar
+// complex buffer (plane stride n*n) in place: d[n], e[n-1] real, tau[n-1]
+// complex interleaved
+void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau);
 GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag);
 
 double heff_flops(const dmrg_heff_desc& d);

@@ -77,6 +77,24 @@ __global__ void scale_k(double* __restrict__ X, int rows, int cols, long long pl
   }
 }
 
+// dst (interleaved complex) = src (real), n elements
+__global__ void real_to_inter_k(const double* __restrict__ src, long long n, double* __restrict__ dst) {
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  for (; i < n; i += (long long)gridDim.x * 256) {
+    dst[2 * i] = src[i];
+    dst[2 * i + 1] = 0.0;
+  }
+}
+
+// dst (planar, plane stride n) = conj(src (interleaved)), n elements
+__global__ void inter_to_planar_conj_k(const double* __restrict__ src, long long n, double* __restrict__ dst) {
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  for (; i < n; i += (long long)gridDim.x * 256) {
+    dst[i] = src[2 * i];
+    dst[i + n] = -src[2 * i + 1];
+  }
+}
+
 __global__ void planar_to_inter_k(const double* __restrict__ src, long long n, double* __restrict__ dst) {
   long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   for (; i < n; i += (long long)gridDim.x * 256) {
@@ -431,7 +449,15 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
     }
     d.pA = mn; d.pB = mn; d.pC = (long long)g * g;
     gemm_f64(c, d, Ms, Ms, Gplanar);
-    if (cx) {
+    if (cx && c->eig_mode == 1) {
+      // complex Hermitian: own reduction to a real tridiagonal, rocSOLVER's
+      // divide and conquer on it; unmtr on the kept columns after keep
+      trd_Z = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * ((size_t)g * g + 2 * (size_t)g + 16));
+      trd_tau = trd_Z + (size_t)g * g;
+      hetrd_lower(c, Gplanar, g, ev, ework, trd_tau);
+      check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, g, ev, ework, trd_Z, g, info), "dstedc");
+      trd_refl = Gplanar;
+    } else if (cx) {
       double* Gi = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)g * g);
       planar_to_inter_k<<<g1((long long)g * g), 256, 0, c->stream>>>(Gplanar, (long long)g * g, Gi);
       DMRG_CHECK_HIP(hipGetLastError());
@@ -471,13 +497,30 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
   if (!(svals[0] > 0.0 && svals[keep - 1] >= 1e-7 * svals[0]))
     return svd_full(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals);
   DMRG_REQUIRE(keep <= kcap, "svd: keep exceeds output capacity");
-  if (trd_refl) {
+  if (trd_refl && !cx) {
     // eigenvectors of G for the kept (largest) values only: Q times the
     // last keep eigenvector columns of the tridiagonal
     ProfScope ps(c, 2, 2.0 * (double)g * g * keep);
     check_rb(rocsolver_dormtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, g, keep,
                               trd_refl, g, trd_tau, trd_Z + (size_t)(g - keep) * g, g),
              "dormtr");
+  } else if (trd_refl) {
+    // complex: the kept tridiagonal eigenvectors as complex columns, the
+    // reflectors interleaved, then Q (complex) times them
+    ProfScope ps(c, 2, 8.0 * (double)g * g * keep);
+    double* Zc = (double*)ws_get(c, WS_TRD_ZC, sizeof(double) * 2 * (size_t)g * g);
+    double* Gi = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)g * g);
+    double* Zk = Zc + 2 * (size_t)(g - keep) * g;
+    real_to_inter_k<<<g1((long long)keep * g), 256, 0, c->stream>>>(trd_Z + (size_t)(g - keep) * g,
+                                                                      (long long)keep * g, Zk);
+    DMRG_CHECK_HIP(hipGetLastError());
+    planar_to_inter_k<<<g1((long long)g * g), 256, 0, c->stream>>>(trd_refl, (long long)g * g, Gi);
+    DMRG_CHECK_HIP(hipGetLastError());
+    check_rb(rocsolver_zunmtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, g, keep,
+                              (rocblas_double_complex*)Gi, g, (rocblas_double_complex*)trd_tau,
+                              (rocblas_double_complex*)Zk, g),
+             "zunmtr");
+    Gsolve = Zc;
   }
   // device copy of the kept singular values
   double* sdev = ev;  // reuse: ev no longer needed
@@ -873,6 +916,47 @@ int sym_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
   DMRG_CHECK_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
   DMRG_REQUIRE(hinfo == 0, "sym_eig: eigensolver did not converge");
+  return 0;
+}
+
+// All eigenpairs of a Hermitian matrix A (row-major, planar complex, plane
+// stride n*n): w ascending, Z planar with Z[k][:] the eigenvector of w[k].
+// mode 1: own reduction + stedc + unmtr, mode 0: heevd.  A is overwritten.
+// The column-major view of the buffer is conj(A); its eigenvectors come out
+// as rows of the interleaved buffer and are conjugated on the way out.
+int herm_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
+  DMRG_REQUIRE(n >= 2, "herm_eig: order must be at least 2");
+  const long long nn = (long long)n * n;
+  double* ework = (double*)ws_get(c, WS_SVD_EV, sizeof(double) * ((size_t)3 * n + 16));
+  double* tauv = ework + n;   // complex, interleaved
+  rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
+  double* Zc = (double*)ws_get(c, WS_TRD_ZC, sizeof(double) * 2 * (size_t)nn);
+  double* Ai = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)nn);
+  if (mode == 1) {
+    double* Zr = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * (size_t)nn);
+    hetrd_lower(c, A, n, w, ework, tauv);
+    check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, n, w, ework, Zr, n, info), "dstedc");
+    real_to_inter_k<<<g1(nn), 256, 0, c->stream>>>(Zr, nn, Zc);
+    DMRG_CHECK_HIP(hipGetLastError());
+    planar_to_inter_k<<<g1(nn), 256, 0, c->stream>>>(A, nn, Ai);
+    DMRG_CHECK_HIP(hipGetLastError());
+    check_rb(rocsolver_zunmtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n,
+                              (rocblas_double_complex*)Ai, n, (rocblas_double_complex*)tauv,
+                              (rocblas_double_complex*)Zc, n),
+             "zunmtr");
+  } else {
+    planar_to_inter_k<<<g1(nn), 256, 0, c->stream>>>(A, nn, Zc);
+    DMRG_CHECK_HIP(hipGetLastError());
+    check_rb(rocsolver_zheevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, (rocblas_double_complex*)Zc,
+                              n, w, ework, info),
+             "zheevd");
+  }
+  inter_to_planar_conj_k<<<g1(nn), 256, 0, c->stream>>>(Zc, nn, Z);
+  DMRG_CHECK_HIP(hipGetLastError());
+  int hinfo = 0;
+  DMRG_CHECK_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  DMRG_REQUIRE(hinfo == 0, "herm_eig: eigensolver did not converge");
   return 0;
 }
 

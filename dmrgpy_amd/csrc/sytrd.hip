@@ -361,4 +361,446 @@ void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
                                 hipMemcpyDeviceToDevice, c->stream));
 }
 
+
+// ---------------------------------------------------------------------------
+// Complex Hermitian reduction (LAPACK zhetrd, lower, on the column-major view
+// B of the row-major buffer: buf[r][c] = B(c, r), so row j of the buffer is
+// column j of B and the matrix-vector product reads conj(buf)).  Same two
+// launches per column as the real reduction, on planar complex data (real
+// plane, then imaginary plane).  Reflector j: beta = -sign(Re a) |a_{j+1:}|,
+// tau = ((beta - Re a0) / beta, -Im a0 / beta), v = a_{j+2:} / (a0 - beta)
+// (zlarfg), except that an exactly zero sub-column with a real a0 takes the
+// reflection tau = 2, v = e_{j+1}; d and e are real, tau complex
+// (interleaved, as rocSOLVER's zunmtr takes it).  The panel update is
+// A22 -= V W^H + W V^H.
+// ---------------------------------------------------------------------------
+constexpr int TRC_PS = 1 + 4 * TRD_NB;   // |a|^2, W^H a and V^H a (re, im) per workgroup
+
+struct TrcState {
+  double alr, ali;   // alpha of the previous column
+  double tr, ti;     // tau of the previous column
+};
+
+__global__ __launch_bounds__(256) void trc_col_k(const double* __restrict__ A, int n, int j, int jj,
+                                                 const double* __restrict__ V, double* __restrict__ W,
+                                                 const double* __restrict__ wpre, double* __restrict__ avec,
+                                                 double* __restrict__ part1, const double* __restrict__ part2,
+                                                 int g2prev, TrcState* __restrict__ st) {
+  __shared__ double sVr[TRD_ROWS1][TRD_NB + 1], sVi[TRD_ROWS1][TRD_NB + 1];
+  __shared__ double sWr[TRD_ROWS1][TRD_NB + 1], sWi[TRD_ROWS1][TRD_NB + 1];
+  __shared__ double sWjr[TRD_NB], sWji[TRD_NB], sVjr[TRD_NB], sVji[TRD_NB], sh[8];
+  __shared__ double sar[TRD_ROWS1], sai[TRD_ROWS1];
+  __shared__ double redr[256], redi[256];
+  const int tid = threadIdx.x;
+  const size_t pA = (size_t)n * n, pV = (size_t)n * TRD_NB;
+  const int r0 = j + blockIdx.x * TRD_ROWS1;
+  const int nr = max(0, min(TRD_ROWS1, n - r0));
+  const int r = r0 + tid;
+  // ---- loads
+  const double tr = st->tr, ti = st->ti;
+  double psr = 0.0, psi = 0.0;   // this thread's share of sum conj(w) v of column j-1
+  if (jj > 0)
+    for (int b = tid; b < g2prev; b += 256) {
+      psr += part2[2 * b];
+      psi += part2[2 * b + 1];
+    }
+  constexpr int SPT = TRD_ROWS1 * TRD_NB / 256;
+  double lvr[SPT], lvi[SPT], lwr[SPT], lwi[SPT];
+#pragma unroll
+  for (int u = 0; u < SPT; ++u) {
+    const int idx = tid + 256 * u, k = idx % TRD_NB;
+    lvr[u] = lvi[u] = lwr[u] = lwi[u] = 0.0;
+    if (idx < nr * TRD_NB && k < jj) {
+      const size_t o = (size_t)r0 * TRD_NB + idx;
+      lvr[u] = V[o];
+      lvi[u] = V[o + pV];
+      lwr[u] = W[o];
+      lwi[u] = W[o + pV];
+    }
+  }
+  double vjr = 0.0, vji = 0.0, wjr = 0.0, wji = 0.0;
+  if (tid < jj) {
+    const size_t o = (size_t)j * TRD_NB + tid;
+    vjr = V[o];
+    vji = V[o + pV];
+    if (tid == jj - 1) {
+      wjr = wpre[j];
+      wji = wpre[j + n];
+    } else {
+      wjr = W[o];
+      wji = W[o + pV];
+    }
+  }
+  double arr = 0.0, ari = 0.0, wpr = 0.0, wpi = 0.0;
+  if (tid < nr) {
+    arr = A[(size_t)j * n + r];
+    ari = A[(size_t)j * n + r + pA];
+    if (jj > 0) {
+      wpr = wpre[r];
+      wpi = wpre[r + n];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < SPT; ++u) {
+    const int idx = tid + 256 * u;
+    sVr[idx / TRD_NB][idx % TRD_NB] = lvr[u];
+    sVi[idx / TRD_NB][idx % TRD_NB] = lvi[u];
+    sWr[idx / TRD_NB][idx % TRD_NB] = lwr[u];
+    sWi[idx / TRD_NB][idx % TRD_NB] = lwi[u];
+  }
+  // ---- alpha of column j-1: -tau/2 sum conj(w) v
+  if (jj > 0) {
+    psr = wave_sum(psr);
+    psi = wave_sum(psi);
+    if ((tid & 63) == 0) {
+      redr[tid >> 6] = psr;
+      redi[tid >> 6] = psi;
+    }
+  }
+  __syncthreads();
+  double alr = 0.0, ali = 0.0;
+  if (jj > 0) {
+    const double sr = redr[0] + redr[1] + redr[2] + redr[3];
+    const double si = redi[0] + redi[1] + redi[2] + redi[3];
+    alr = -0.5 * (tr * sr - ti * si);
+    ali = -0.5 * (tr * si + ti * sr);
+    if (blockIdx.x == 0 && tid == 0) {
+      st->alr = alr;
+      st->ali = ali;
+    }
+  }
+  if (tid < jj) {
+    sVjr[tid] = vjr;
+    sVji[tid] = vji;
+    if (tid == jj - 1) {
+      sWjr[tid] = wjr + alr * vjr - ali * vji;
+      sWji[tid] = wji + alr * vji + ali * vjr;
+    } else {
+      sWjr[tid] = wjr;
+      sWji[tid] = wji;
+    }
+  }
+  if (jj > 0 && tid < nr) {   // finish column jj-1 of W: w + alpha v
+    const double xr = sVr[tid][jj - 1], xi = sVi[tid][jj - 1];
+    const double wr = wpr + alr * xr - ali * xi, wi = wpi + alr * xi + ali * xr;
+    sWr[tid][jj - 1] = wr;
+    sWi[tid][jj - 1] = wi;
+    W[(size_t)r * TRD_NB + jj - 1] = wr;
+    W[(size_t)r * TRD_NB + jj - 1 + pV] = wi;
+  }
+  __syncthreads();
+  // ---- column j of B, updated with the panel: a -= V conj(W_j)^T + W conj(V_j)^T
+  double pn = 0.0;
+  if (tid < TRD_ROWS1) sar[tid] = sai[tid] = 0.0;
+  if (tid < nr) {
+    double ar = arr, ai = ari;
+    for (int k = 0; k < jj; ++k) {
+      // V[r][k] conj(Wj[k]) + W[r][k] conj(Vj[k])
+      ar -= sVr[tid][k] * sWjr[k] + sVi[tid][k] * sWji[k] + sWr[tid][k] * sVjr[k] + sWi[tid][k] * sVji[k];
+      ai -= sVi[tid][k] * sWjr[k] - sVr[tid][k] * sWji[k] + sWi[tid][k] * sVjr[k] - sWr[tid][k] * sVji[k];
+    }
+    avec[r] = ar;
+    avec[r + n] = ai;
+    if (r >= j + 2) {
+      sar[tid] = ar;
+      sai[tid] = ai;
+      pn = ar * ar + ai * ai;
+    }
+  }
+  const double pns = block_sum(pn, sh);
+  {  // partial conj(W)^T a (k < jj) and conj(V)^T a (jj <= k < 2 jj) over this block's rows
+    const int k = tid & 63, grp = tid >> 6;
+    double accr = 0.0, acci = 0.0;
+    if (k < 2 * jj)
+      for (int i = grp; i < nr; i += 4) {
+        const double xr = k < jj ? sWr[i][k] : sVr[i][k - jj];
+        const double xi = k < jj ? sWi[i][k] : sVi[i][k - jj];
+        accr += xr * sar[i] + xi * sai[i];
+        acci += xr * sai[i] - xi * sar[i];
+      }
+    redr[grp * 64 + k] = accr;
+    redi[grp * 64 + k] = acci;
+  }
+  __syncthreads();
+  double* myp = part1 + (size_t)blockIdx.x * TRC_PS;
+  if (tid == 0) myp[0] = pns;
+  if (tid < 2 * jj) {
+    const double re = redr[tid] + redr[64 + tid] + redr[128 + tid] + redr[192 + tid];
+    const double im = redi[tid] + redi[64 + tid] + redi[128 + tid] + redi[192 + tid];
+    const int base = tid < jj ? 1 + tid : 1 + 2 * jj + (tid - jj);   // W^H a: [1, 1+2jj), V^H a: [1+2jj, 1+4jj)
+    myp[base] = re;
+    myp[base + jj] = im;
+  }
+}
+
+__global__ __launch_bounds__(256) void trc_symv_k(double* __restrict__ A, int n, int j, int jj,
+                                                  double* __restrict__ V, const double* __restrict__ W,
+                                                  double* __restrict__ wpre, const double* __restrict__ avec,
+                                                  const double* __restrict__ part1, int g1,
+                                                  double* __restrict__ part2, TrcState* __restrict__ st,
+                                                  double* __restrict__ d, double* __restrict__ e,
+                                                  double* __restrict__ tau) {
+  __shared__ double syr[TRD_NB], syi[TRD_NB], szr[TRD_NB], szi[TRD_NB];
+  __shared__ double redr[4][TRD_ROWS2], redi[4][TRD_ROWS2];
+  __shared__ double red4[256];
+  __shared__ double tot[TRC_PS];
+  __shared__ double sst[4];   // tau (re, im), scale (re, im)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const size_t pA = (size_t)n * n, pV = (size_t)n * TRD_NB;
+  const int nk = 1 + 4 * jj;   // <= 125
+  const int rbase = j + 1 + blockIdx.x * TRD_ROWS2;
+  const int nr = min(TRD_ROWS2, n - rbase);   // >= 1
+  // ---- loads: launch-1 partials (two groups of 128), reflector inputs, correction rows
+  double ps = 0.0;
+  {
+    const int k = tid & 127, grp = tid >> 7;
+    if (k < nk) {
+#pragma unroll 4
+      for (int b = grp; b < g1; b += 2) ps += part1[(size_t)b * TRC_PS + k];
+    }
+  }
+  double a0r = 0.0, a0i = 0.0, aj = 0.0;
+  if (tid == 0) {
+    a0r = avec[j + 1];
+    a0i = avec[j + 1 + n];
+    aj = avec[j];
+  }
+  double v1r = 0.0, v1i = 0.0, w1r = 0.0, w1i = 0.0;
+  if (tid < jj) {
+    const size_t o = (size_t)(j + 1) * TRD_NB + tid;
+    v1r = V[o];
+    v1i = V[o + pV];
+    if (tid == jj - 1) {
+      const double alr = st->alr, ali = st->ali;
+      w1r = wpre[j + 1] + alr * v1r - ali * v1i;
+      w1i = wpre[j + 1 + n] + alr * v1i + ali * v1r;
+    } else {
+      w1r = W[o];
+      w1i = W[o + pV];
+    }
+  }
+  double mr[TRD_ROWS2], mi[TRD_ROWS2];
+#pragma unroll
+  for (int i = 0; i < TRD_ROWS2; ++i) mr[i] = mi[i] = 0.0;
+  if (tid < 64 && (lane & 31) < jj) {
+    const double* M = lane < 32 ? V : W;
+#pragma unroll
+    for (int i = 0; i < TRD_ROWS2; ++i)
+      if (i < nr) {
+        const size_t o = (size_t)(rbase + i) * TRD_NB + (lane & 31);
+        mr[i] = M[o];
+        mi[i] = M[o + pV];
+      }
+  }
+  // ---- the matrix stream: acc = sum_{c >= j+2} conj(buf[r][c]) a[c], a1 = conj(buf[r][j+1])
+  double accr[TRD_ROWS2], acci[TRD_ROWS2], a1r[TRD_ROWS2], a1i[TRD_ROWS2];
+#pragma unroll
+  for (int i = 0; i < TRD_ROWS2; ++i) accr[i] = acci[i] = a1r[i] = a1i[i] = 0.0;
+#pragma unroll 4
+  for (int c = j + 1 + tid; c < n; c += 256) {
+    const bool first = c == j + 1;
+    const double ar = first ? 0.0 : avec[c], ai = first ? 0.0 : avec[c + n];
+#pragma unroll
+    for (int i = 0; i < TRD_ROWS2; ++i) {
+      const int r = rbase + min(i, nr - 1);   // rows past the end repeat the last (discarded)
+      const double xr = A[(size_t)r * n + c], xi = A[(size_t)r * n + c + pA];
+      accr[i] += xr * ar + xi * ai;
+      acci[i] += xr * ai - xi * ar;
+      if (first) {
+        a1r[i] = xr;
+        a1i[i] = -xi;
+      }
+    }
+  }
+  // ---- reflector (every workgroup, same fixed-order sums)
+  red4[tid] = ps;
+  __syncthreads();
+  if (tid < nk) tot[tid] = red4[tid] + red4[128 + tid];
+  __syncthreads();
+  if (tid == 0) {
+    const double xn2 = tot[0];
+    // an exactly zero sub-column with a real a0 takes tau = 2, v = e_{j+1}
+    // (H = diag(-1, 1, ...), beta = -a0) instead of LAPACK's tau = 0:
+    // rocSOLVER's complex back-transformation (unmtr) returns NaN vectors
+    // when some but not all reflectors have tau = 0
+    double tr = 2.0, ti = 0.0, beta = -a0r, scr = 0.0, sci = 0.0;
+    if (xn2 != 0.0 || a0i != 0.0) {
+      beta = -copysign(hypot(hypot(a0r, a0i), sqrt(xn2)), a0r);
+      tr = (beta - a0r) / beta;
+      ti = -a0i / beta;
+      // scale = 1 / (a0 - beta)
+      const double dr = a0r - beta, di = a0i;
+      const double den = dr * dr + di * di;
+      scr = dr / den;
+      sci = -di / den;
+    }
+    sst[0] = tr;
+    sst[1] = ti;
+    sst[2] = scr;
+    sst[3] = sci;
+    if (blockIdx.x == 0) {
+      d[j] = aj;
+      e[j] = beta;
+      tau[2 * j] = tr;
+      tau[2 * j + 1] = ti;
+      st->tr = tr;
+      st->ti = ti;
+    }
+  }
+  __syncthreads();
+  const double tr = sst[0], ti = sst[1], scr = sst[2], sci = sst[3];
+  if (tid < jj) {   // y = W^H v = conj(W[j+1]) + scale W^H a, z = V^H v likewise
+    const double war = tot[1 + tid], wai = tot[1 + jj + tid];
+    const double var = tot[1 + 2 * jj + tid], vai = tot[1 + 3 * jj + tid];
+    syr[tid] = w1r + scr * war - sci * wai;
+    syi[tid] = -w1i + scr * wai + sci * war;
+    szr[tid] = v1r + scr * var - sci * vai;
+    szi[tid] = -v1i + scr * vai + sci * var;
+  }
+  __syncthreads();
+  // ---- p - V y - W z per row
+  double fr = 0.0, fi = 0.0;
+  if (tid < 64 && (lane & 31) < jj) {
+    fr = lane < 32 ? syr[lane & 31] : szr[lane & 31];
+    fi = lane < 32 ? syi[lane & 31] : szi[lane & 31];
+  }
+#pragma unroll
+  for (int i = 0; i < TRD_ROWS2; ++i) {
+    const double qr = wave_sum(a1r[i] + scr * accr[i] - sci * acci[i] - (mr[i] * fr - mi[i] * fi));
+    const double qi = wave_sum(a1i[i] + scr * acci[i] + sci * accr[i] - (mr[i] * fi + mi[i] * fr));
+    if (lane == 0) {
+      redr[wave][i] = qr;
+      redi[wave][i] = qi;
+    }
+  }
+  __syncthreads();
+  if (tid < TRD_ROWS2) {
+    double pr = 0.0, pi = 0.0;
+    if (tid < nr) {
+      const int r = rbase + tid;
+      const double qr = redr[0][tid] + redr[1][tid] + redr[2][tid] + redr[3][tid];
+      const double qi = redi[0][tid] + redi[1][tid] + redi[2][tid] + redi[3][tid];
+      const double wr = tr * qr - ti * qi, wi = tr * qi + ti * qr;
+      double vr = 1.0, vi = 0.0;
+      if (r != j + 1) {
+        const double ar = avec[r], ai = avec[r + n];
+        vr = scr * ar - sci * ai;
+        vi = scr * ai + sci * ar;
+      }
+      wpre[r] = wr;
+      wpre[r + n] = wi;
+      V[(size_t)r * TRD_NB + jj] = vr;
+      V[(size_t)r * TRD_NB + jj + pV] = vi;
+      if (r >= j + 2) {   // reflector storage (row j)
+        A[(size_t)j * n + r] = vr;
+        A[(size_t)j * n + r + pA] = vi;
+      }
+      pr = wr * vr + wi * vi;   // conj(w) v
+      pi = wr * vi - wi * vr;
+    }
+    red4[tid] = pr;
+    red4[8 + tid] = pi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sr = 0.0, si = 0.0;
+    for (int i = 0; i < TRD_ROWS2; ++i) {
+      sr += red4[i];
+      si += red4[8 + i];
+    }
+    part2[2 * blockIdx.x] = sr;
+    part2[2 * blockIdx.x + 1] = si;
+  }
+}
+
+__global__ __launch_bounds__(256) void trc_fin_k(int n, int s, int k, const double* __restrict__ V,
+                                                 double* __restrict__ W, const double* __restrict__ wpre,
+                                                 const double* __restrict__ part2, int g2,
+                                                 const TrcState* __restrict__ st) {
+  __shared__ double red4[256];
+  double psr = 0.0, psi = 0.0;
+  for (int b = threadIdx.x; b < g2; b += 256) {
+    psr += part2[2 * b];
+    psi += part2[2 * b + 1];
+  }
+  psr = wave_sum(psr);
+  psi = wave_sum(psi);
+  if ((threadIdx.x & 63) == 0) {
+    red4[threadIdx.x >> 6] = psr;
+    red4[4 + (threadIdx.x >> 6)] = psi;
+  }
+  __syncthreads();
+  const double sr = red4[0] + red4[1] + red4[2] + red4[3], si = red4[4] + red4[5] + red4[6] + red4[7];
+  const double tr = st->tr, ti = st->ti;
+  const double alr = -0.5 * (tr * sr - ti * si), ali = -0.5 * (tr * si + ti * sr);
+  const size_t pV = (size_t)n * TRD_NB;
+  const int r = s + blockIdx.x * 256 + threadIdx.x;
+  if (r < n) {
+    const size_t o = (size_t)r * TRD_NB + k;
+    const double vr = V[o], vi = V[o + pV];
+    W[o] = wpre[r] + alr * vr - ali * vi;
+    W[o + pV] = wpre[r + n] + alr * vi + ali * vr;
+  }
+}
+
+// A (n x n, planar complex with plane stride n*n, Hermitian; the buffer read
+// column-major) -> d[n], e[n-1] (real), tau[n-1] (complex, interleaved) and
+// reflectors in place (zhetrd, uplo 'L', on the column-major view).
+void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
+  DMRG_REQUIRE(n >= 2, "hetrd: order must be at least 2");
+  const size_t vw = (size_t)n * TRD_NB;
+  const int gmax1 = (n + TRD_ROWS1 - 1) / TRD_ROWS1;
+  const int gmax2 = (n + TRD_ROWS2 - 1) / TRD_ROWS2;
+  double* buf = (double*)ws_get(c, WS_TRD, sizeof(double) * (4 * vw + 4 * (size_t)n + (size_t)gmax1 * TRC_PS +
+                                                              2 * (size_t)gmax2 + 8) +
+                                               sizeof(TrcState) + 64);
+  double* V = buf;             // planar: re plane, im plane (stride vw)
+  double* W = V + 2 * vw;
+  double* wpre = W + 2 * vw;   // planar, stride n
+  double* avec = wpre + 2 * (size_t)n;
+  double* part1 = avec + 2 * (size_t)n;
+  double* part2 = part1 + (size_t)gmax1 * TRC_PS;
+  TrcState* st = reinterpret_cast<TrcState*>(part2 + 2 * (size_t)gmax2 + 8);
+  DMRG_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(TrcState), c->stream));
+  // algorithmic bytes: the lower triangle of the trailing matrix (both planes) once per column
+  double bytes = 0.0;
+  for (int j = 0; j < n - 1; ++j) bytes += 16.0 * (double)(n - j - 1) * (n - j) / 2.0;
+  ProfScope ps(c, 4, 0.0, bytes);
+  int g2prev = 1;
+  for (int j0 = 0; j0 < n - 1; j0 += TRD_NB) {
+    const int nbp = std::min(TRD_NB, n - 1 - j0);
+    for (int jj = 0; jj < nbp; ++jj) {
+      const int j = j0 + jj;
+      const int g1 = (n - j + TRD_ROWS1 - 1) / TRD_ROWS1;
+      const int g2 = (n - j - 1 + TRD_ROWS2 - 1) / TRD_ROWS2;
+      DMRG_REQUIRE(g1 >= 1 && g1 <= gmax1 && g2 >= 1 && g2 <= gmax2, "hetrd: grid exceeds partial buffers");
+      trc_col_k<<<g1, 256, 0, c->stream>>>(A, n, j, jj, V, W, wpre, avec, part1, part2, g2prev, st);
+      DMRG_CHECK_HIP(hipGetLastError());
+      trc_symv_k<<<g2, 256, 0, c->stream>>>(A, n, j, jj, V, W, wpre, avec, part1, g1, part2, st, d, e, tau);
+      DMRG_CHECK_HIP(hipGetLastError());
+      g2prev = g2;
+    }
+    const int s = j0 + nbp;
+    if (s < n) {
+      trc_fin_k<<<(n - s + 255) / 256, 256, 0, c->stream>>>(n, s, nbp - 1, V, W, wpre, part2, g2prev, st);
+      DMRG_CHECK_HIP(hipGetLastError());
+      // buf(r, c) -= conj(W[r]) . V[c] + conj(V[r]) . W[c]   (B22 -= V W^H + W V^H)
+      const int m = n - s;
+      GemmDesc g = gdesc(m, m, nbp, TRD_NB, TRD_NB, n, CPLX, 4);
+      g.transB = true;
+      g.conjA = true;
+      g.alpha = -1.0;
+      g.beta = 1.0;
+      g.pA = (long long)vw;
+      g.pB = (long long)vw;
+      g.pC = (long long)n * n;
+      double* C = A + (size_t)s * n + s;
+      gemm_f64(c, g, W + (size_t)s * TRD_NB, V + (size_t)s * TRD_NB, C);
+      gemm_f64(c, g, V + (size_t)s * TRD_NB, W + (size_t)s * TRD_NB, C);
+    }
+  }
+  DMRG_CHECK_HIP(hipMemcpyAsync(d + (n - 1), A + (size_t)(n - 1) * n + (n - 1), sizeof(double),
+                                hipMemcpyDeviceToDevice, c->stream));
+}
+
 }  // namespace dmrg

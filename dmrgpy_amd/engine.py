@@ -213,9 +213,21 @@ class Engine:
         return U, V, k, disc.value, sv
 
     def set_eigensolver(self, mode):
-        """1: own tridiagonalisation + rocSOLVER stedc/ormtr (default);
-        0: rocSOLVER syevd -- for the Gram-route splits."""
+        """1: own tridiagonalisation (real or Hermitian) + rocSOLVER
+        stedc/ormtr/unmtr (default); 0: rocSOLVER syevd/heevd -- for the
+        Gram-route splits."""
         check(self.lib.dmrg_set_eigensolver(self.ctx, int(mode)), "dmrg_set_eigensolver")
+
+    def herm_eig(self, A, mode=1):
+        """Eigenpairs of a Hermitian device matrix given as planar float64
+        [2, n, n] (copied): (w ascending [n], Z [2, n, n] planar with row k
+        the eigenvector of w[k])."""
+        n = A.shape[-1]
+        Ac = A.reshape(2, n, n).clone()
+        w = self.empty((n,))
+        Z = self.empty((2, n, n))
+        check(self.lib.dmrg_herm_eig(self.ctx, int(mode), _ptr(Ac), n, _ptr(w), _ptr(Z)), "dmrg_herm_eig")
+        return w, Z
 
     def sym_eig(self, A, mode=1):
         """Eigenpairs of a symmetric float64 device matrix [n, n] (copied):

@@ -149,13 +149,13 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
                           double* V, int kcap, int* keep, double* discarded, double* svals_host, int* used,
                           double* resid);
 
-/* Dense symmetric eigensolver of the real Gram route (the np.linalg.eigh
- * call of _svd_truncated, src/dmrgpy/pyitensor/svd.py:134): mode 1 (the
- * default) reduces the matrix to tridiagonal form with this library's
- * two-launch-per-column Householder kernels, then runs rocSOLVER's divide
- * and conquer on the tridiagonal and back-transforms only the eigenvectors
- * the truncation keeps; mode 0 calls rocSOLVER dsyevd.  Complex Gram
- * matrices always use zheevd. */
+/* Dense eigensolver of the Gram route (the np.linalg.eigh call of
+ * _svd_truncated, src/dmrgpy/pyitensor/svd.py:134): mode 1 (the default)
+ * reduces the real symmetric or complex Hermitian Gram matrix to a real
+ * tridiagonal with this library's two-launch-per-column Householder
+ * kernels, then runs rocSOLVER's divide and conquer on the tridiagonal and
+ * back-transforms (ormtr / unmtr) only the eigenvectors the truncation
+ * keeps; mode 0 calls rocSOLVER dsyevd / zheevd. */
 int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode);
 
 /* All eigenpairs of the real symmetric n x n matrix A (device, row-major;
@@ -163,6 +163,15 @@ int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode);
  * in dmrg_set_eigensolver.  The eigensolver of dmrg_svd_split exposed for
  * its parity tests; replaces np.linalg.eigh (svd.py:134). */
 int dmrg_sym_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z);
+
+/* All eigenpairs of the Hermitian n x n matrix A (device, row-major, planar
+ * complex: real plane then imaginary plane, plane stride n*n; overwritten):
+ * w ascending, row k of Z (planar, same layout) the eigenvector of w[k].
+ * mode 1: this library's reduction to a real tridiagonal (zhetrd
+ * conventions), rocSOLVER dstedc and zunmtr; mode 0: rocSOLVER zheevd.  The
+ * eigensolver of the complex dmrg_svd_split exposed for its parity tests;
+ * replaces np.linalg.eigh on the complex Gram matrix (svd.py:134). */
+int dmrg_herm_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z);
 
 /* Gradient of <bra|H|ket> with respect to one MPO site tensor W[wl][wr][s'][s]:
  * G = sum L[wl][a'][a] conj(bra[a'][s'][b']) ket[a][s][b] R[wr][b][b'], written

@@ -1,7 +1,9 @@
-"""Dense symmetric eigensolver of the Gram route (dmrg_sym_eig /
+"""Dense eigensolvers of the Gram route (dmrg_sym_eig, dmrg_herm_eig,
 dmrg_set_eigensolver): the library's own two-launch-per-column Householder
-tridiagonalisation followed by rocSOLVER's stedc/ormtr (mode 1, the
-default), against numpy.linalg.eigh (LAPACK) and rocSOLVER's syevd (mode 0).
+tridiagonalisation (real symmetric, and complex Hermitian to a real
+tridiagonal) followed by rocSOLVER's stedc/ormtr/unmtr (mode 1, the
+default), against numpy.linalg.eigh (LAPACK) and rocSOLVER's syevd/heevd
+(mode 0).
 
 Orders cover a single column block (n < 32), panel boundaries (32, 33, 64,
 65), ragged sizes and the Gram shapes of the sweeps; matrices include
@@ -113,3 +115,124 @@ def test_svd_split_modes_agree(engine):
     skeep = s0_[k0_ - 1]
     assert np.linalg.norm(P0 - P1) < (1e-12 + 100 * eps / skeep) * np.linalg.norm(P0)
     assert np.abs(U1_.T @ U1_ - np.eye(k1_)).max() < 1e-12
+
+
+# ---- complex Hermitian (dmrg_herm_eig): the own reduction to a real
+# tridiagonal (zhetrd conventions) + stedc + zunmtr (mode 1), zheevd (mode 0)
+
+def _check_herm(engine, A, mode, tol_rel=1e-13):
+    import torch
+    n = A.shape[0]
+    w, Z = engine.herm_eig(_dev(engine, np.stack([A.real, A.imag])), mode)
+    torch.cuda.synchronize()
+    w, Z = w.cpu().numpy(), Z.cpu().numpy()
+    Z = Z[0] + 1j * Z[1]
+    ref = np.linalg.eigvalsh(A)
+    scale = max(np.abs(ref).max(), 1e-300)
+    assert np.all(np.diff(w) >= -1e-12 * scale)
+    assert np.abs(w - ref).max() <= tol_rel * scale * max(1.0, np.sqrt(n) / 4)
+    # rows of Z are eigenvectors of A (not of conj(A)): residual, orthonormality
+    R = A @ Z.T - Z.T * w
+    assert np.abs(R).max() <= 1e-12 * scale * np.sqrt(n)
+    assert np.abs(Z.conj() @ Z.T - np.eye(n)).max() <= 1e-12 * np.sqrt(n)
+    return w
+
+
+@pytest.mark.parametrize("n", [2, 3, 17, 32, 33, 64, 65, 100, 257])
+@pytest.mark.parametrize("mode", [1, 0])
+def test_random_hermitian(engine, n, mode):
+    rng = np.random.default_rng(n + 100 * mode + 7)
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    _check_herm(engine, X + X.conj().T, mode)
+
+
+@pytest.mark.parametrize("n", [96, 390])
+def test_hermitian_gram_decaying_spectrum(engine, n):
+    """Complex PSD Gram matrix with eigenvalues 1 .. 1e-14: the own
+    reduction and zheevd agree to 1e-13 of the largest value."""
+    rng = np.random.default_rng(5 * n)
+    Q = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+    lam = np.logspace(0, -14, n)
+    A = (Q * lam) @ Q.conj().T
+    A = (A + A.conj().T) / 2
+    w1 = _check_herm(engine, A, 1, tol_rel=2e-14)
+    w0 = _check_herm(engine, A, 0, tol_rel=2e-14)
+    assert np.abs(w1 - w0).max() <= 1e-13
+
+
+def test_hermitian_special(engine):
+    """Block-diagonal matrices (exactly zero sub-columns: the reduction takes
+    the reflection tau = 2 there, since rocSOLVER's zunmtr returns NaN
+    vectors when some reflectors have tau = 0 -- zheevd itself does on these
+    matrices), a real symmetric matrix given as complex, and a diagonal."""
+    rng = np.random.default_rng(9)
+    n = 70
+    A = np.zeros((n, n), complex)
+    for b0, b1 in [(0, 10), (10, 45), (45, 46), (46, 70)]:
+        X = rng.standard_normal((b1 - b0, b1 - b0)) + 1j * rng.standard_normal((b1 - b0, b1 - b0))
+        A[b0:b1, b0:b1] = X + X.conj().T
+    A[11, 10] = 0.5j   # a lone complex entry right below the diagonal
+    A[10, 11] = -0.5j
+    _check_herm(engine, A, 1)
+    X = rng.standard_normal((40, 40))
+    _check_herm(engine, (X + X.T).astype(complex), 1)
+    _check_herm(engine, np.diag(rng.standard_normal(50)).astype(complex), 1)
+
+
+def test_svd_split_complex_modes_agree(engine):
+    """Complex dmrg_svd_split with the own Hermitian reduction and with
+    zheevd: same keep and discarded weight, values and truncated product
+    within the Gram-route rounding bound."""
+    import torch
+    rng = np.random.default_rng(13)
+    m, n, k = 192, 160, 160
+    U0 = np.linalg.qr(rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k)))[0]
+    V0 = np.linalg.qr(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k)))[0]
+    s0 = np.logspace(0, -8, k)
+    M = (U0 * s0) @ V0.conj().T
+    Md = _dev(engine, np.stack([M.real, M.imag]))
+    out = {}
+    try:
+        for mode in (0, 1):
+            engine.set_eigensolver(mode)
+            U, V, keep, disc, sv = engine.svd_split(1, Md, m, n, 1e-12, 0, 1, 0)
+            torch.cuda.synchronize()
+            Uc = U.cpu().numpy()
+            Vc = V.cpu().numpy()
+            out[mode] = (Uc[0] + 1j * Uc[1], Vc[0] + 1j * Vc[1], keep, disc, sv)
+    finally:
+        engine.set_eigensolver(1)
+    U0_, V0_, k0_, d0_, s0_ = out[0]
+    U1_, V1_, k1_, d1_, s1_ = out[1]
+    assert k0_ == k1_
+    assert abs(d0_ - d1_) < 1e-15
+    eps = 2.2e-16
+    assert np.all(np.abs(s0_ - s1_) <= 1e-13 + 8 * eps / np.maximum(s0_, 1e-300))
+    P0, P1 = U0_ @ V0_, U1_ @ V1_
+    skeep = s0_[k0_ - 1]
+    assert np.linalg.norm(P0 - P1) < (1e-12 + 100 * eps / skeep) * np.linalg.norm(P0)
+    assert np.abs(U1_.conj().T @ U1_ - np.eye(k1_)).max() < 1e-12
+    assert np.linalg.norm(P1 - M) <= 1e-6 * np.linalg.norm(M)
+
+
+def test_svd_split_complex_block_structure(engine):
+    """A complex two-site matrix with conserved-charge block structure (its
+    Gram matrix is block-diagonal: exactly zero sub-columns): the default
+    split returns finite factors that reproduce M to the discarded weight."""
+    import torch
+    rng = np.random.default_rng(17)
+    m, n = 96, 80
+    M = np.zeros((m, n), complex)
+    for (r0, r1), (c0, c1) in [((0, 30), (0, 20)), ((30, 70), (20, 60)), ((70, 96), (60, 80))]:
+        k = min(r1 - r0, c1 - c0)
+        Ub = np.linalg.qr(rng.standard_normal((r1 - r0, k)) + 1j * rng.standard_normal((r1 - r0, k)))[0]
+        Vb = np.linalg.qr(rng.standard_normal((c1 - c0, k)) + 1j * rng.standard_normal((c1 - c0, k)))[0]
+        M[r0:r1, c0:c1] = (Ub * np.logspace(0, -5, k)) @ Vb.conj().T
+    U, V, keep, disc, sv = engine.svd_split(1, _dev(engine, np.stack([M.real, M.imag])), m, n, 1e-12, 0, 1, 0)
+    torch.cuda.synchronize()
+    Uc, Vc = U.cpu().numpy(), V.cpu().numpy()
+    assert np.all(np.isfinite(Uc)) and np.all(np.isfinite(Vc))
+    P = (Uc[0] + 1j * Uc[1]) @ (Vc[0] + 1j * Vc[1])
+    ref = np.linalg.svd(M, compute_uv=False)
+    assert np.abs(sv[:keep] - ref[:keep]).max() <= 1e-13 + 8 * 2.2e-16 / ref[keep - 1]
+    assert np.linalg.norm(P - M) <= (1.01 * np.sqrt(disc) + 1e-11) * np.linalg.norm(M)
