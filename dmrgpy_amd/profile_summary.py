@@ -97,7 +97,8 @@ def calibration(path, counter, nbytes):
 
 
 CATEGORIES = (
-    ("trd", "own Householder tridiagonalisation (trd_col_k, trd_symv_k, trd_fin_k; its panel GEMMs count as SVD GEMMs)"),
+    ("trd", "own Householder tridiagonalisation (real: trd_col_k, trd_symv_k, trd_fin_k; complex Hermitian: "
+            "trc_col_k, trc_symv_k, trc_fin_k; its panel GEMMs count as SVD GEMMs)"),
     ("eigensolver", "rocSOLVER/rocBLAS: tridiagonal divide and conquer (stedc), back-transformation (ormtr), "
                     "syevd/heevd where used, and the library GEMMs they call"),
     ("heff", "H_eff GEMMs (L- and R-step)"),
@@ -113,7 +114,7 @@ CATEGORIES = (
 
 
 def _category(name):
-    if "trd_" in name:
+    if "trd_" in name or "trc_" in name:
         return "trd"
     if "rocsolver" in name or "rocblas" in name or name.startswith("Cijk_"):
         return "eigensolver"

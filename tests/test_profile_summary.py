@@ -20,6 +20,7 @@ def test_roles_and_categories():
     assert ps._role("_ZN4dmrg16smallop_sparse_kIfLb0EEEvPKT_") == "small-op"
     assert ps._role("void dmrg::gemm_fast<double, 4, 2, false, true, 4>(double const*)") == "gemm-other"
     assert ps._category("dmrg::trd_symv_k(double*, int)") == "trd"
+    assert ps._category("dmrg::trc_col_k(double const*, int)") == "trd"
     assert ps._category("void rocsolver::v33001::stedc_solve_kernel<double>(int)") == "eigensolver"
     assert ps._category("Cijk_Alik_Bljk_DB_MT128x64x16") == "eigensolver"
     assert ps._category("void dmrg::gemm_fast<float, 2, 2, true, false, 3>(float const*)") == "env"
