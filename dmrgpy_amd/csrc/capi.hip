@@ -224,7 +224,9 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
 int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode) {
   API_BEGIN
   DMRG_REQUIRE(ctx, "set_eigensolver: null context");
-  DMRG_REQUIRE(mode == 0 || mode == 1, "set_eigensolver: mode must be 0 (rocSOLVER syevd) or 1 (own tridiagonalisation)");
+  DMRG_REQUIRE(mode >= 0 && mode <= 2,
+               "set_eigensolver: mode must be 0 (rocSOLVER syevd), 1 (own tridiagonalisation and back-"
+               "transformation) or 2 (own tridiagonalisation, rocSOLVER ormtr)");
   ctx->c.eig_mode = mode;
   API_END
 }
@@ -232,7 +234,7 @@ int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode) {
 int dmrg_sym_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z) {
   API_BEGIN
   DMRG_REQUIRE(ctx && A && w && Z, "sym_eig: null argument");
-  DMRG_REQUIRE(mode == 0 || mode == 1, "sym_eig: mode must be 0 or 1");
+  DMRG_REQUIRE(mode >= 0 && mode <= 2, "sym_eig: mode must be 0, 1 or 2");
   dmrg::sym_eig_entry(&ctx->c, mode, A, n, w, Z);
   API_END
 }
@@ -240,7 +242,7 @@ int dmrg_sym_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z
 int dmrg_herm_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z) {
   API_BEGIN
   DMRG_REQUIRE(ctx && A && w && Z, "herm_eig: null argument");
-  DMRG_REQUIRE(mode == 0 || mode == 1, "herm_eig: mode must be 0 or 1");
+  DMRG_REQUIRE(mode >= 0 && mode <= 2, "herm_eig: mode must be 0, 1 or 2");
   dmrg::herm_eig_entry(&ctx->c, mode, A, n, w, Z);
   API_END
 }

@@ -75,7 +75,7 @@ enum Slot {
   WS_SVD_GRAM, WS_SVD_EV, WS_SVD_TMP, WS_SVD_TMP2, WS_SVD_INFO, WS_SVD_CPLX, WS_SVD_FIX, WS_SVD_RED, WS_PROJ, WS_B3_L, WS_B3_R,
   WS_BF16_A, WS_BF16_B, WS_REFINE,
   WS_GSV_Q, WS_GSV_B, WS_GSV_R, WS_GSV_S, WS_GSV_V, WS_GSV_T, WS_GSV_T2, WS_GSV_OU, WS_GSV_OV,
-  WS_TRD, WS_TRD_Z, WS_TRD_ZC,
+  WS_TRD, WS_TRD_Z, WS_TRD_ZC, WS_BT_V, WS_BT_W,
   WS_NSLOTS
 };
 
@@ -103,9 +103,10 @@ struct Ctx {
   long long prof_calls[5] = {0, 0, 0, 0, 0};
   long long matvecs = 0;
   // dense eigensolver of the Gram route: 1 = own two-launch
-  // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc /
-  // ormtr or unmtr (eigenvectors back-transformed only for the kept
-  // values), 0 = rocSOLVER syevd / heevd
+  // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc + the
+  // back-transformation of the kept eigenvectors (own blocked GEMMs for real
+  // matrices, rocSOLVER unmtr for complex), 2 = the same with rocSOLVER's
+  // ormtr for real matrices, 0 = rocSOLVER syevd / heevd
   int eig_mode = 1;
 };
 

@@ -67,6 +67,10 @@ void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau);
 // complex buffer (plane stride n*n) in place: d[n], e[n-1] real, tau[n-1]
 // complex interleaved
 void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau);
+// rows of Zt (keep x n, row-major: eigenvectors of the tridiagonal) <- rows
+// of Q Y for the reflectors sytrd_lower left in A (dormtr L/lower/N by
+// blocked compact-WY GEMMs)
+void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Zt, int keep);
 GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag);
 
 double heff_flops(const dmrg_heff_desc& d);

@@ -449,7 +449,7 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
     }
     d.pA = mn; d.pB = mn; d.pC = (long long)g * g;
     gemm_f64(c, d, Ms, Ms, Gplanar);
-    if (cx && c->eig_mode == 1) {
+    if (cx && c->eig_mode >= 1) {
       // complex Hermitian: own reduction to a real tridiagonal, rocSOLVER's
       // divide and conquer on it; unmtr on the kept columns after keep
       trd_Z = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * ((size_t)g * g + 2 * (size_t)g + 16));
@@ -465,7 +465,7 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
                                 g, ev, ework, info),
                "zheevd");
       Gsolve = Gi;
-    } else if (c->eig_mode == 1 && g >= 32) {
+    } else if (c->eig_mode >= 1 && g >= 32) {
       // tridiagonalise here, eigenpairs of the tridiagonal by rocSOLVER's
       // divide and conquer; the back-transformation waits for keep
       trd_Z = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * ((size_t)g * g + g + 16));
@@ -501,9 +501,12 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
     // eigenvectors of G for the kept (largest) values only: Q times the
     // last keep eigenvector columns of the tridiagonal
     ProfScope ps(c, 2, 2.0 * (double)g * g * keep);
-    check_rb(rocsolver_dormtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, g, keep,
-                              trd_refl, g, trd_tau, trd_Z + (size_t)(g - keep) * g, g),
-             "dormtr");
+    if (c->eig_mode == 1)
+      ormtr_lower_rows(c, trd_refl, g, trd_tau, trd_Z + (size_t)(g - keep) * g, keep);
+    else
+      check_rb(rocsolver_dormtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, g, keep,
+                                trd_refl, g, trd_tau, trd_Z + (size_t)(g - keep) * g, g),
+               "dormtr");
   } else if (trd_refl) {
     // complex: the kept tridiagonal eigenvectors as complex columns, the
     // reflectors interleaved, then Q (complex) times them
@@ -894,19 +897,22 @@ int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cut
 
 // All eigenpairs of a real symmetric matrix (row-major = column-major):
 // w ascending, Z[k][:] the eigenvector of w[k] (rows of the row-major Z).
-// mode 1: own tridiagonalisation + stedc + ormtr, mode 0: syevd.  A is
-// overwritten.
+// mode 1: own tridiagonalisation + stedc + own back-transformation, mode 2:
+// the same with rocSOLVER's ormtr, mode 0: syevd.  A is overwritten.
 int sym_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
   DMRG_REQUIRE(n >= 2, "sym_eig: order must be at least 2");
   double* ework = (double*)ws_get(c, WS_SVD_EV, sizeof(double) * ((size_t)2 * n + 16));
   double* tauv = ework + n;
   rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
-  if (mode == 1) {
+  if (mode == 1 || mode == 2) {
     sytrd_lower(c, A, n, w, ework, tauv);
     check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, n, w, ework, Z, n, info), "dstedc");
-    check_rb(rocsolver_dormtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, A, n,
-                              tauv, Z, n),
-             "dormtr");
+    if (mode == 1)
+      ormtr_lower_rows(c, A, n, tauv, Z, n);
+    else
+      check_rb(rocsolver_dormtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, A, n,
+                                tauv, Z, n),
+               "dormtr");
   } else {
     check_rb(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, n, w, ework, info),
              "dsyevd");
@@ -921,7 +927,8 @@ int sym_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
 
 // All eigenpairs of a Hermitian matrix A (row-major, planar complex, plane
 // stride n*n): w ascending, Z planar with Z[k][:] the eigenvector of w[k].
-// mode 1: own reduction + stedc + unmtr, mode 0: heevd.  A is overwritten.
+// mode 1 (or 2): own reduction + stedc + unmtr, mode 0: heevd.  A is
+// overwritten.
 // The column-major view of the buffer is conj(A); its eigenvectors come out
 // as rows of the interleaved buffer and are conjugated on the way out.
 int herm_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
@@ -932,7 +939,7 @@ int herm_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
   rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
   double* Zc = (double*)ws_get(c, WS_TRD_ZC, sizeof(double) * 2 * (size_t)nn);
   double* Ai = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)nn);
-  if (mode == 1) {
+  if (mode >= 1) {
     double* Zr = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * (size_t)nn);
     hetrd_lower(c, A, n, w, ework, tauv);
     check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, n, w, ework, Zr, n, info), "dstedc");

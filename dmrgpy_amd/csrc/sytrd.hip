@@ -803,4 +803,117 @@ void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
                                 hipMemcpyDeviceToDevice, c->stream));
 }
 
+
+// ---------------------------------------------------------------------------
+// Back-transformation of kept eigenvectors (dormtr, side L, lower, no
+// transpose, on the reflectors sytrd_lower leaves): E = Q Y with
+// Q = H_0 H_1 ... H_{n-2}, H_j = I - tau_j v_j v_j^T.  The reflectors are
+// grouped into blocks of BT_NB (compact WY: Q_b = I - V_b T_b V_b^T, T_b upper
+// triangular, LAPACK dlarft forward/columnwise), and each block is applied
+// to the eigenvectors as three GEMMs, last block first.  The eigenvectors
+// are the rows of Zt (keep x n, row-major), so a block is Zt <- Zt Q_b^T:
+//   Y = Zt V_b (split along its long inner dimension into BT_SPLIT partial
+//   products written side by side), Y2 = [Y_0 .. Y_{S-1}] [T_b^T; ..; T_b^T],
+//   Zt -= Y2 V_b^T.
+// ---------------------------------------------------------------------------
+constexpr int BT_NB = 128;
+constexpr int BT_SPLIT = 8;
+
+// Vall[j][r] = v_j[r] for rows j < n - 1 (0 for r <= j, 1 at r = j + 1, the
+// stored entry A[j][r] beyond), zero rows up to the block multiple
+__global__ void bt_vall_k(const double* __restrict__ A, int n, long long total, double* __restrict__ Vall) {
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long long)gridDim.x * 256) {
+    const int j = (int)(idx / n), r = (int)(idx % n);
+    double v = 0.0;
+    if (j < n - 1) {
+      if (r == j + 1) v = 1.0;
+      else if (r > j + 1) v = A[(size_t)j * n + r];
+    }
+    Vall[idx] = v;
+  }
+}
+
+// T_b from S_b = V_b V_b^T and tau (one workgroup per block, thread k owns
+// row k of T: T[k][k] = tau_k, T[k][i] = -tau_i sum_{l=k}^{i-1} T[k][l] S[l][i]),
+// then BT_SPLIT stacked copies of T_b^T
+__global__ __launch_bounds__(BT_NB) void bt_larft_k(const double* __restrict__ S, const double* __restrict__ tau,
+                                                   int n, double* __restrict__ TtS) {
+  __shared__ double T[BT_NB][BT_NB + 1];
+  __shared__ double sS[BT_NB];
+  const int b = blockIdx.x, k = threadIdx.x;
+  const int j0 = b * BT_NB;
+  const double* Sb = S + (size_t)b * BT_NB * BT_NB;
+  for (int i = 0; i < BT_NB; ++i) T[k][i] = 0.0;
+  for (int i = 0; i < BT_NB; ++i) {
+    sS[k] = Sb[(size_t)k * BT_NB + i];   // column i of S
+    __syncthreads();
+    const double ti = j0 + i < n - 1 ? tau[j0 + i] : 0.0;
+    if (k < i) {
+      double acc = 0.0;
+      for (int l = k; l < i; ++l) acc += T[k][l] * sS[l];
+      T[k][i] = -ti * acc;
+    } else if (k == i) {
+      T[k][k] = ti;
+    }
+    __syncthreads();
+  }
+  double* out = TtS + (size_t)b * BT_SPLIT * BT_NB * BT_NB;
+  for (int s = 0; s < BT_SPLIT; ++s)
+    for (int i = 0; i < BT_NB; ++i) out[((size_t)s * BT_NB + k) * BT_NB + i] = T[i][k];
+}
+
+// Zt (keep x n, row-major, the rows eigenvectors of the tridiagonal) <- rows
+// of Q Y, Q from the reflectors in A (row-major n x n as sytrd_lower leaves
+// it) and tau
+void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Zt, int keep) {
+  DMRG_REQUIRE(n >= 2 && keep >= 1 && keep <= n, "ormtr: bad sizes");
+  const int nblk = (n - 1 + BT_NB - 1) / BT_NB;
+  const size_t vrows = (size_t)nblk * BT_NB;
+  double* Vall = (double*)ws_get(c, WS_BT_V, sizeof(double) * vrows * n);
+  const size_t nS = (size_t)nblk * BT_NB * BT_NB, nT = (size_t)nblk * BT_SPLIT * BT_NB * BT_NB;
+  const size_t nY = (size_t)keep * BT_SPLIT * BT_NB, nY2 = (size_t)keep * BT_NB;
+  double* Sm = (double*)ws_get(c, WS_BT_W, sizeof(double) * (nS + nT + nY + nY2));
+  double* TtS = Sm + nS;
+  double* Ycat = TtS + nT;
+  double* Y2 = Ycat + nY;
+  const long long total = (long long)vrows * n;
+  bt_vall_k<<<(int)std::min<long long>((total + 255) / 256, 65536), 256, 0, c->stream>>>(A, n, total, Vall);
+  DMRG_CHECK_HIP(hipGetLastError());
+  {  // S_b = V_b V_b^T for every block at once
+    GemmDesc g = gdesc(BT_NB, BT_NB, n, n, n, BT_NB, REAL, 4);
+    g.transB = true;
+    g.batch = nblk;
+    g.sA = g.sB = (long long)BT_NB * n;
+    g.sC = (long long)BT_NB * BT_NB;
+    gemm_f64(c, g, Vall, Vall, Sm);
+  }
+  bt_larft_k<<<nblk, BT_NB, 0, c->stream>>>(Sm, tau, n, TtS);
+  DMRG_CHECK_HIP(hipGetLastError());
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int b0 = b * BT_NB;
+    const int len = n - b0 - 1;   // columns b0 + 1 .. n - 1 carry block b
+    if (len <= 0) continue;
+    int S = BT_SPLIT, kc = len, start = b0 + 1;
+    for (; S >= 1; --S) {   // S equal chunks ending at column n; columns left of b0 + 1 meet zero rows of V_b
+      kc = (len + S - 1) / S;
+      start = n - S * kc;
+      if (start >= 0) break;
+    }
+    DMRG_REQUIRE(S >= 1 && start >= 0 && start <= b0 + 1, "ormtr: bad split");
+    const double* Vb = Vall + (size_t)b * BT_NB * n;
+    GemmDesc g1 = gdesc(keep, BT_NB, kc, n, n, S * BT_NB, REAL, 4);
+    g1.transB = true;
+    g1.batch = S;
+    g1.sA = g1.sB = kc;
+    g1.sC = BT_NB;
+    gemm_f64(c, g1, Zt + start, Vb + start, Ycat);
+    GemmDesc g2 = gdesc(keep, BT_NB, S * BT_NB, S * BT_NB, BT_NB, BT_NB, REAL, 4);
+    gemm_f64(c, g2, Ycat, TtS + (size_t)b * BT_SPLIT * BT_NB * BT_NB, Y2);
+    GemmDesc g3 = gdesc(keep, len, BT_NB, BT_NB, n, n, REAL, 4);
+    g3.alpha = -1.0;
+    g3.beta = 1.0;
+    gemm_f64(c, g3, Y2, Vb + b0 + 1, Zt + b0 + 1);
+  }
+}
+
 }  // namespace dmrg

@@ -153,9 +153,11 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
  * _svd_truncated, src/dmrgpy/pyitensor/svd.py:134): mode 1 (the default)
  * reduces the real symmetric or complex Hermitian Gram matrix to a real
  * tridiagonal with this library's two-launch-per-column Householder
- * kernels, then runs rocSOLVER's divide and conquer on the tridiagonal and
- * back-transforms (ormtr / unmtr) only the eigenvectors the truncation
- * keeps; mode 0 calls rocSOLVER dsyevd / zheevd. */
+ * kernels, runs rocSOLVER's divide and conquer on the tridiagonal and
+ * back-transforms only the eigenvectors the truncation keeps (real: this
+ * library's blocked compact-WY GEMMs; complex: rocSOLVER zunmtr); mode 2 is
+ * mode 1 with rocSOLVER dormtr for the real back-transformation; mode 0
+ * calls rocSOLVER dsyevd / zheevd. */
 int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode);
 
 /* All eigenpairs of the real symmetric n x n matrix A (device, row-major;

@@ -46,8 +46,8 @@ def _check(engine, A, mode, tol_rel=1e-13):
     return w
 
 
-@pytest.mark.parametrize("n", [2, 3, 17, 32, 33, 64, 65, 100, 257])
-@pytest.mark.parametrize("mode", [1, 0])
+@pytest.mark.parametrize("n", [2, 3, 17, 32, 33, 64, 65, 100, 127, 128, 129, 130, 257, 300])
+@pytest.mark.parametrize("mode", [1, 2, 0])
 def test_random_symmetric(engine, n, mode):
     rng = np.random.default_rng(n + 100 * mode)
     X = rng.standard_normal((n, n))
@@ -68,6 +68,29 @@ def test_gram_decaying_spectrum(engine, n):
     assert np.abs(w1 - w0).max() <= 1e-13
 
 
+@pytest.mark.parametrize("n", [300, 1100])
+def test_back_transformation_matches_ormtr(engine, n):
+    """The blocked compact-WY back-transformation (mode 1) against rocSOLVER
+    dormtr (mode 2) after the same reduction and divide and conquer: equal
+    eigenvectors to rounding (several 128-reflector blocks, split inner
+    products), on a Gram-like spectrum."""
+    import torch
+    rng = np.random.default_rng(n + 3)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    lam = np.maximum(1e-9 ** (np.arange(n) / 256.0), 1e-17)
+    A = (Q * lam) @ Q.T
+    A = (A + A.T) / 2
+    out = {}
+    for mode in (1, 2):
+        w, Z = engine.sym_eig(_dev(engine, A), mode)
+        torch.cuda.synchronize()
+        out[mode] = (w.cpu().numpy(), Z.cpu().numpy())
+    assert np.array_equal(out[1][0], out[2][0])   # same reduction and stedc
+    assert np.abs(out[1][1] - out[2][1]).max() <= 1e-12 * np.sqrt(n)
+    Z = out[1][1]
+    assert np.abs(Z @ Z.T - np.eye(n)).max() <= 1e-12 * np.sqrt(n)
+
+
 def test_zero_subcolumns(engine):
     """Block-diagonal and diagonal matrices: Householder vectors of zero
     norm (tau = 0) in many columns."""
@@ -81,7 +104,8 @@ def test_zero_subcolumns(engine):
     _check(engine, np.diag(rng.standard_normal(70)), 1)
 
 
-def test_svd_split_modes_agree(engine):
+@pytest.mark.parametrize("other", [1, 2])
+def test_svd_split_modes_agree(engine, other):
     """dmrg_svd_split with the two eigensolvers on a DMRG-like two-site
     matrix (spectrum decaying to 1e-8): same keep count and discarded
     weight, singular values and truncated product within the Gram-route
@@ -96,7 +120,7 @@ def test_svd_split_modes_agree(engine):
     Md = _dev(engine, M[None])
     out = {}
     try:
-        for mode in (0, 1):
+        for mode in (0, other):
             engine.set_eigensolver(mode)
             U, V, keep, disc, sv = engine.svd_split(0, Md, m, n, 1e-12, 0, 1, 0)
             torch.cuda.synchronize()
@@ -104,7 +128,7 @@ def test_svd_split_modes_agree(engine):
     finally:
         engine.set_eigensolver(1)
     U0_, V0_, k0_, d0_, s0_ = out[0]
-    U1_, V1_, k1_, d1_, s1_ = out[1]
+    U1_, V1_, k1_, d1_, s1_ = out[other]
     assert k0_ == k1_
     assert abs(d0_ - d1_) < 1e-15
     # Gram route: s comes from an eigenvalue known to ~eps*s_max^2, i.e. to
