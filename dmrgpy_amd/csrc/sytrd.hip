@@ -835,31 +835,40 @@ __global__ void bt_vall_k(const double* __restrict__ A, int n, long long total, 
 
 // T_b from S_b = V_b V_b^T and tau (one workgroup per block, thread k owns
 // row k of T: T[k][k] = tau_k, T[k][i] = -tau_i sum_{l=k}^{i-1} T[k][l] S[l][i]),
-// then BT_SPLIT stacked copies of T_b^T
+// then BT_SPLIT stacked copies of T_b^T.  The strict upper triangle of S
+// (column-packed) and the upper triangle of T (row-packed) sit in LDS, so
+// the recurrence runs without global loads or barriers.
+constexpr int BT_TRI = BT_NB * (BT_NB + 1) / 2;
+__device__ __forceinline__ int bt_s_idx(int l, int i) { return i * (i - 1) / 2 + l; }            // l < i
+__device__ __forceinline__ int bt_t_idx(int k, int l) { return k * BT_NB - k * (k - 1) / 2 + (l - k); }  // l >= k
+
 __global__ __launch_bounds__(BT_NB) void bt_larft_k(const double* __restrict__ S, const double* __restrict__ tau,
                                                    int n, double* __restrict__ TtS) {
-  __shared__ double T[BT_NB][BT_NB + 1];
-  __shared__ double sS[BT_NB];
+  __shared__ double sS[BT_TRI];
+  __shared__ double sT[BT_TRI];
+  __shared__ double stau[BT_NB];
   const int b = blockIdx.x, k = threadIdx.x;
   const int j0 = b * BT_NB;
   const double* Sb = S + (size_t)b * BT_NB * BT_NB;
-  for (int i = 0; i < BT_NB; ++i) T[k][i] = 0.0;
-  for (int i = 0; i < BT_NB; ++i) {
-    sS[k] = Sb[(size_t)k * BT_NB + i];   // column i of S
-    __syncthreads();
-    const double ti = j0 + i < n - 1 ? tau[j0 + i] : 0.0;
-    if (k < i) {
-      double acc = 0.0;
-      for (int l = k; l < i; ++l) acc += T[k][l] * sS[l];
-      T[k][i] = -ti * acc;
-    } else if (k == i) {
-      T[k][k] = ti;
-    }
-    __syncthreads();
+  stau[k] = j0 + k < n - 1 ? tau[j0 + k] : 0.0;
+#pragma unroll 16
+  for (int t = 0; t < BT_NB; ++t) {   // S[t][k], kept for t < k (column k of the strict upper triangle)
+    const double v = Sb[(size_t)t * BT_NB + k];
+    if (t < k) sS[bt_s_idx(t, k)] = v;
   }
+  __syncthreads();
+  sT[bt_t_idx(k, k)] = stau[k];
+  for (int i = k + 1; i < BT_NB; ++i) {
+    const double ti = stau[i];
+    double acc = 0.0;
+    for (int l = k; l < i; ++l) acc += sT[bt_t_idx(k, l)] * sS[bt_s_idx(l, i)];
+    sT[bt_t_idx(k, i)] = -ti * acc;
+  }
+  __syncthreads();
   double* out = TtS + (size_t)b * BT_SPLIT * BT_NB * BT_NB;
   for (int s = 0; s < BT_SPLIT; ++s)
-    for (int i = 0; i < BT_NB; ++i) out[((size_t)s * BT_NB + k) * BT_NB + i] = T[i][k];
+    for (int i = 0; i < BT_NB; ++i)   // row k of T^T = column k of T
+      out[((size_t)s * BT_NB + k) * BT_NB + i] = i <= k ? sT[bt_t_idx(i, k)] : 0.0;
 }
 
 // Zt (keep x n, row-major, the rows eigenvectors of the tridiagonal) <- rows
