@@ -103,10 +103,9 @@ struct Ctx {
   long long prof_calls[5] = {0, 0, 0, 0, 0};
   long long matvecs = 0;
   // dense eigensolver of the Gram route: 1 = own two-launch
-  // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc + the
-  // back-transformation of the kept eigenvectors (own blocked GEMMs for real
-  // matrices, rocSOLVER unmtr for complex), 2 = the same with rocSOLVER's
-  // ormtr for real matrices, 0 = rocSOLVER syevd / heevd
+  // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc + own
+  // blocked back-transformation of the kept eigenvectors, 2 = the same with
+  // rocSOLVER's ormtr / unmtr, 0 = rocSOLVER syevd / heevd
   int eig_mode = 1;
 };
 

@@ -71,6 +71,9 @@ void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau);
 // of Q Y for the reflectors sytrd_lower left in A (dormtr L/lower/N by
 // blocked compact-WY GEMMs)
 void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Zt, int keep);
+// complex: rows of Et (keep x n, planar, plane stride keep*n) <- rows of Q Y
+// for hetrd_lower's reflectors (zunmtr L/lower/N)
+void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Et, int keep);
 GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag);
 
 double heff_flops(const dmrg_heff_desc& d);

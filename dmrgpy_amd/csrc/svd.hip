@@ -508,21 +508,31 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
                                 trd_refl, g, trd_tau, trd_Z + (size_t)(g - keep) * g, g),
                "dormtr");
   } else if (trd_refl) {
-    // complex: the kept tridiagonal eigenvectors as complex columns, the
-    // reflectors interleaved, then Q (complex) times them
+    // complex: Q (complex) times the kept tridiagonal eigenvectors, then
+    // interleaved columns as the heevd route leaves them
     ProfScope ps(c, 2, 8.0 * (double)g * g * keep);
     double* Zc = (double*)ws_get(c, WS_TRD_ZC, sizeof(double) * 2 * (size_t)g * g);
-    double* Gi = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)g * g);
     double* Zk = Zc + 2 * (size_t)(g - keep) * g;
-    real_to_inter_k<<<g1((long long)keep * g), 256, 0, c->stream>>>(trd_Z + (size_t)(g - keep) * g,
-                                                                      (long long)keep * g, Zk);
-    DMRG_CHECK_HIP(hipGetLastError());
-    planar_to_inter_k<<<g1((long long)g * g), 256, 0, c->stream>>>(trd_refl, (long long)g * g, Gi);
-    DMRG_CHECK_HIP(hipGetLastError());
-    check_rb(rocsolver_zunmtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, g, keep,
-                              (rocblas_double_complex*)Gi, g, (rocblas_double_complex*)trd_tau,
-                              (rocblas_double_complex*)Zk, g),
-             "zunmtr");
+    const long long kg = (long long)keep * g;
+    if (c->eig_mode == 1) {   // own blocked product on planar rows
+      double* Et = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)kg);
+      DMRG_CHECK_HIP(hipMemcpyAsync(Et, trd_Z + (size_t)(g - keep) * g, sizeof(double) * kg,
+                                    hipMemcpyDeviceToDevice, c->stream));
+      DMRG_CHECK_HIP(hipMemsetAsync(Et + kg, 0, sizeof(double) * kg, c->stream));
+      unmtr_lower_rows(c, trd_refl, g, trd_tau, Et, keep);
+      planar_to_inter_k<<<g1(kg), 256, 0, c->stream>>>(Et, kg, Zk);
+      DMRG_CHECK_HIP(hipGetLastError());
+    } else {   // rocSOLVER zunmtr on interleaved reflectors
+      double* Gi = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)g * g);
+      real_to_inter_k<<<g1(kg), 256, 0, c->stream>>>(trd_Z + (size_t)(g - keep) * g, kg, Zk);
+      DMRG_CHECK_HIP(hipGetLastError());
+      planar_to_inter_k<<<g1((long long)g * g), 256, 0, c->stream>>>(trd_refl, (long long)g * g, Gi);
+      DMRG_CHECK_HIP(hipGetLastError());
+      check_rb(rocsolver_zunmtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, g, keep,
+                                (rocblas_double_complex*)Gi, g, (rocblas_double_complex*)trd_tau,
+                                (rocblas_double_complex*)Zk, g),
+               "zunmtr");
+    }
     Gsolve = Zc;
   }
   // device copy of the kept singular values
@@ -927,8 +937,8 @@ int sym_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
 
 // All eigenpairs of a Hermitian matrix A (row-major, planar complex, plane
 // stride n*n): w ascending, Z planar with Z[k][:] the eigenvector of w[k].
-// mode 1 (or 2): own reduction + stedc + unmtr, mode 0: heevd.  A is
-// overwritten.
+// mode 1: own reduction + stedc + own back-transformation, mode 2: the same
+// with rocSOLVER's unmtr, mode 0: heevd.  A is overwritten.
 // The column-major view of the buffer is conj(A); its eigenvectors come out
 // as rows of the interleaved buffer and are conjugated on the way out.
 int herm_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
@@ -943,14 +953,22 @@ int herm_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
     double* Zr = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * (size_t)nn);
     hetrd_lower(c, A, n, w, ework, tauv);
     check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, n, w, ework, Zr, n, info), "dstedc");
-    real_to_inter_k<<<g1(nn), 256, 0, c->stream>>>(Zr, nn, Zc);
-    DMRG_CHECK_HIP(hipGetLastError());
-    planar_to_inter_k<<<g1(nn), 256, 0, c->stream>>>(A, nn, Ai);
-    DMRG_CHECK_HIP(hipGetLastError());
-    check_rb(rocsolver_zunmtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n,
-                              (rocblas_double_complex*)Ai, n, (rocblas_double_complex*)tauv,
-                              (rocblas_double_complex*)Zc, n),
-             "zunmtr");
+    if (mode == 1) {   // Ai holds the planar rows of Q Y
+      DMRG_CHECK_HIP(hipMemcpyAsync(Ai, Zr, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
+      DMRG_CHECK_HIP(hipMemsetAsync(Ai + nn, 0, sizeof(double) * nn, c->stream));
+      unmtr_lower_rows(c, A, n, tauv, Ai, n);
+      planar_to_inter_k<<<g1(nn), 256, 0, c->stream>>>(Ai, nn, Zc);
+      DMRG_CHECK_HIP(hipGetLastError());
+    } else {
+      real_to_inter_k<<<g1(nn), 256, 0, c->stream>>>(Zr, nn, Zc);
+      DMRG_CHECK_HIP(hipGetLastError());
+      planar_to_inter_k<<<g1(nn), 256, 0, c->stream>>>(A, nn, Ai);
+      DMRG_CHECK_HIP(hipGetLastError());
+      check_rb(rocsolver_zunmtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n,
+                                (rocblas_double_complex*)Ai, n, (rocblas_double_complex*)tauv,
+                                (rocblas_double_complex*)Zc, n),
+               "zunmtr");
+    }
   } else {
     planar_to_inter_k<<<g1(nn), 256, 0, c->stream>>>(A, nn, Zc);
     DMRG_CHECK_HIP(hipGetLastError());

@@ -925,4 +925,147 @@ void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Complex counterpart (zunmtr L/lower/N on hetrd_lower's reflectors), planar
+// complex throughout, blocks of BTC_NB reflectors: Q_b = I - V_b T_b V_b^H
+// (T_b(k, i) = -tau_i sum_l T_b(k, l) v_l^H v_i), applied to the rows of
+// Et = (Q Y)^T as Et <- Et - (Et conj(V_b)) T_b^T V_b^T, last block first.
+// ---------------------------------------------------------------------------
+constexpr int BTC_NB = 64;
+constexpr int BTC_TRI = BTC_NB * (BTC_NB + 1) / 2;
+
+__global__ void btc_vall_k(const double* __restrict__ A, int n, long long total, double* __restrict__ Vall) {
+  const size_t pA = (size_t)n * n;
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long long)gridDim.x * 256) {
+    const int j = (int)(idx / n), r = (int)(idx % n);
+    double vr = 0.0, vi = 0.0;
+    if (j < n - 1) {
+      if (r == j + 1) {
+        vr = 1.0;
+      } else if (r > j + 1) {
+        vr = A[(size_t)j * n + r];
+        vi = A[(size_t)j * n + r + pA];
+      }
+    }
+    Vall[idx] = vr;
+    Vall[idx + total] = vi;
+  }
+}
+
+__device__ __forceinline__ int btc_s_idx(int l, int i) { return i * (i - 1) / 2 + l; }
+__device__ __forceinline__ int btc_t_idx(int k, int l) { return k * BTC_NB - k * (k - 1) / 2 + (l - k); }
+
+// S planar (plane stride pS), tau interleaved; TtS planar (plane stride pT)
+__global__ __launch_bounds__(BTC_NB) void btc_larft_k(const double* __restrict__ S, long long pS,
+                                                     const double* __restrict__ tau, int n,
+                                                     double* __restrict__ TtS, long long pT) {
+  __shared__ double sSr[BTC_TRI], sSi[BTC_TRI], sTr[BTC_TRI], sTi[BTC_TRI];
+  __shared__ double str[BTC_NB], sti[BTC_NB];
+  const int b = blockIdx.x, k = threadIdx.x;
+  const int j0 = b * BTC_NB;
+  const double* Sb = S + (size_t)b * BTC_NB * BTC_NB;
+  str[k] = j0 + k < n - 1 ? tau[2 * (j0 + k)] : 0.0;
+  sti[k] = j0 + k < n - 1 ? tau[2 * (j0 + k) + 1] : 0.0;
+#pragma unroll 16
+  for (int t = 0; t < BTC_NB; ++t) {
+    const double vr = Sb[(size_t)t * BTC_NB + k], vi = Sb[(size_t)t * BTC_NB + k + pS];
+    if (t < k) {
+      sSr[btc_s_idx(t, k)] = vr;
+      sSi[btc_s_idx(t, k)] = vi;
+    }
+  }
+  __syncthreads();
+  sTr[btc_t_idx(k, k)] = str[k];
+  sTi[btc_t_idx(k, k)] = sti[k];
+  for (int i = k + 1; i < BTC_NB; ++i) {
+    double ar = 0.0, ai = 0.0;
+    for (int l = k; l < i; ++l) {
+      const double xr = sTr[btc_t_idx(k, l)], xi = sTi[btc_t_idx(k, l)];
+      const double yr = sSr[btc_s_idx(l, i)], yi = sSi[btc_s_idx(l, i)];
+      ar += xr * yr - xi * yi;
+      ai += xr * yi + xi * yr;
+    }
+    const double tr = str[i], ti = sti[i];
+    sTr[btc_t_idx(k, i)] = -(tr * ar - ti * ai);
+    sTi[btc_t_idx(k, i)] = -(tr * ai + ti * ar);
+  }
+  __syncthreads();
+  double* out = TtS + (size_t)b * BT_SPLIT * BTC_NB * BTC_NB;
+  for (int s = 0; s < BT_SPLIT; ++s)
+    for (int r = 0; r < BTC_NB; ++r) {   // row r of T^T (column r of T), coalesced over i = k
+      const size_t o = ((size_t)s * BTC_NB + r) * BTC_NB + k;
+      out[o] = k <= r ? sTr[btc_t_idx(k, r)] : 0.0;
+      out[o + pT] = k <= r ? sTi[btc_t_idx(k, r)] : 0.0;
+    }
+}
+
+// Et (keep x n planar, plane stride keep*n) <- rows of Q Y for hetrd_lower's
+// reflectors in A (planar, plane stride n*n) and tau (interleaved)
+void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Et, int keep) {
+  DMRG_REQUIRE(n >= 2 && keep >= 1 && keep <= n, "unmtr: bad sizes");
+  const int NB = BTC_NB;
+  const int nblk = (n - 1 + NB - 1) / NB;
+  const size_t vrows = (size_t)nblk * NB;
+  const long long pV = (long long)vrows * n;
+  double* Vall = (double*)ws_get(c, WS_BT_V, sizeof(double) * 2 * (size_t)pV);
+  const size_t nS = (size_t)nblk * NB * NB, nT = (size_t)nblk * BT_SPLIT * NB * NB;
+  const size_t nY = (size_t)keep * BT_SPLIT * NB, nY2 = (size_t)keep * NB;
+  double* Sm = (double*)ws_get(c, WS_BT_W, sizeof(double) * 2 * (nS + nT + nY + nY2));
+  double* TtS = Sm + 2 * nS;
+  double* Ycat = TtS + 2 * nT;
+  double* Y2 = Ycat + 2 * nY;
+  btc_vall_k<<<(int)std::min<long long>((pV + 255) / 256, 65536), 256, 0, c->stream>>>(A, n, pV, Vall);
+  DMRG_CHECK_HIP(hipGetLastError());
+  {  // S_b = V_b^H V_b (rows of Vall are the reflectors): conj(Vall_b) Vall_b^T
+    GemmDesc g = gdesc(NB, NB, n, n, n, NB, CPLX, 4);
+    g.transB = true;
+    g.conjA = true;
+    g.batch = nblk;
+    g.sA = g.sB = (long long)NB * n;
+    g.sC = (long long)NB * NB;
+    g.pA = g.pB = pV;
+    g.pC = (long long)nS;
+    gemm_f64(c, g, Vall, Vall, Sm);
+  }
+  btc_larft_k<<<nblk, NB, 0, c->stream>>>(Sm, (long long)nS, tau, n, TtS, (long long)nT);
+  DMRG_CHECK_HIP(hipGetLastError());
+  const long long pE = (long long)keep * n;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int b0 = b * NB;
+    const int len = n - b0 - 1;
+    if (len <= 0) continue;
+    int S = BT_SPLIT, kc = len, start = b0 + 1;
+    for (; S >= 1; --S) {
+      kc = (len + S - 1) / S;
+      start = n - S * kc;
+      if (start >= 0) break;
+    }
+    DMRG_REQUIRE(S >= 1 && start >= 0 && start <= b0 + 1, "unmtr: bad split");
+    const double* Vb = Vall + (size_t)b * NB * n;
+    GemmDesc g1 = gdesc(keep, NB, kc, n, n, S * NB, CPLX, 4);   // Y = Et conj(V_b)
+    g1.transB = true;
+    g1.conjB = true;
+    g1.batch = S;
+    g1.sA = g1.sB = kc;
+    g1.sC = NB;
+    g1.pA = pE;
+    g1.pB = pV;
+    g1.pC = (long long)nY;
+    gemm_f64(c, g1, Et + start, Vb + start, Ycat);
+    GemmDesc g2 = gdesc(keep, NB, S * NB, S * NB, NB, NB, CPLX, 4);   // Y2 = Y T_b^T
+    g2.pA = (long long)nY;
+    g2.pB = (long long)nT;
+    g2.pC = (long long)nY2;
+    gemm_f64(c, g2, Ycat, TtS + (size_t)b * BT_SPLIT * NB * NB, Y2);
+    GemmDesc g3 = gdesc(keep, len, NB, NB, n, n, CPLX, 4);   // Et -= Y2 V_b^T
+    g3.alpha = -1.0;
+    g3.beta = 1.0;
+    g3.pA = (long long)nY2;
+    g3.pB = pV;
+    g3.pC = pE;
+    gemm_f64(c, g3, Y2, Vb + b0 + 1, Et + b0 + 1);
+  }
+}
+
 }  // namespace dmrg

@@ -214,9 +214,8 @@ class Engine:
 
     def set_eigensolver(self, mode):
         """1: own tridiagonalisation (real or Hermitian) + rocSOLVER stedc +
-        own (real) / rocSOLVER unmtr (complex) back-transformation (default);
-        2: as 1 with rocSOLVER ormtr for real matrices; 0: rocSOLVER
-        syevd/heevd -- for the Gram-route splits."""
+        own back-transformation (default); 2: as 1 with rocSOLVER
+        ormtr/unmtr; 0: rocSOLVER syevd/heevd -- for the Gram-route splits."""
         check(self.lib.dmrg_set_eigensolver(self.ctx, int(mode)), "dmrg_set_eigensolver")
 
     def herm_eig(self, A, mode=1):

@@ -154,10 +154,10 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
  * reduces the real symmetric or complex Hermitian Gram matrix to a real
  * tridiagonal with this library's two-launch-per-column Householder
  * kernels, runs rocSOLVER's divide and conquer on the tridiagonal and
- * back-transforms only the eigenvectors the truncation keeps (real: this
- * library's blocked compact-WY GEMMs; complex: rocSOLVER zunmtr); mode 2 is
- * mode 1 with rocSOLVER dormtr for the real back-transformation; mode 0
- * calls rocSOLVER dsyevd / zheevd. */
+ * back-transforms only the eigenvectors the truncation keeps with this
+ * library's blocked compact-WY GEMMs; mode 2 is mode 1 with rocSOLVER's
+ * dormtr / zunmtr for the back-transformation; mode 0 calls rocSOLVER
+ * dsyevd / zheevd. */
 int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode);
 
 /* All eigenpairs of the real symmetric n x n matrix A (device, row-major;
@@ -170,7 +170,8 @@ int dmrg_sym_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z
  * complex: real plane then imaginary plane, plane stride n*n; overwritten):
  * w ascending, row k of Z (planar, same layout) the eigenvector of w[k].
  * mode 1: this library's reduction to a real tridiagonal (zhetrd
- * conventions), rocSOLVER dstedc and zunmtr; mode 0: rocSOLVER zheevd.  The
+ * conventions), rocSOLVER dstedc and this library's back-transformation;
+ * mode 2: the same with rocSOLVER zunmtr; mode 0: rocSOLVER zheevd.  The
  * eigensolver of the complex dmrg_svd_split exposed for its parity tests;
  * replaces np.linalg.eigh on the complex Gram matrix (svd.py:134). */
 int dmrg_herm_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z);

@@ -162,8 +162,8 @@ def _check_herm(engine, A, mode, tol_rel=1e-13):
     return w
 
 
-@pytest.mark.parametrize("n", [2, 3, 17, 32, 33, 64, 65, 100, 257])
-@pytest.mark.parametrize("mode", [1, 0])
+@pytest.mark.parametrize("n", [2, 3, 17, 32, 33, 64, 65, 66, 100, 129, 257])
+@pytest.mark.parametrize("mode", [1, 2, 0])
 def test_random_hermitian(engine, n, mode):
     rng = np.random.default_rng(n + 100 * mode + 7)
     X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
@@ -203,7 +203,8 @@ def test_hermitian_special(engine):
     _check_herm(engine, np.diag(rng.standard_normal(50)).astype(complex), 1)
 
 
-def test_svd_split_complex_modes_agree(engine):
+@pytest.mark.parametrize("other", [1, 2])
+def test_svd_split_complex_modes_agree(engine, other):
     """Complex dmrg_svd_split with the own Hermitian reduction and with
     zheevd: same keep and discarded weight, values and truncated product
     within the Gram-route rounding bound."""
@@ -217,7 +218,7 @@ def test_svd_split_complex_modes_agree(engine):
     Md = _dev(engine, np.stack([M.real, M.imag]))
     out = {}
     try:
-        for mode in (0, 1):
+        for mode in (0, other):
             engine.set_eigensolver(mode)
             U, V, keep, disc, sv = engine.svd_split(1, Md, m, n, 1e-12, 0, 1, 0)
             torch.cuda.synchronize()
@@ -227,7 +228,7 @@ def test_svd_split_complex_modes_agree(engine):
     finally:
         engine.set_eigensolver(1)
     U0_, V0_, k0_, d0_, s0_ = out[0]
-    U1_, V1_, k1_, d1_, s1_ = out[1]
+    U1_, V1_, k1_, d1_, s1_ = out[other]
     assert k0_ == k1_
     assert abs(d0_ - d1_) < 1e-15
     eps = 2.2e-16
@@ -260,3 +261,26 @@ def test_svd_split_complex_block_structure(engine):
     ref = np.linalg.svd(M, compute_uv=False)
     assert np.abs(sv[:keep] - ref[:keep]).max() <= 1e-13 + 8 * 2.2e-16 / ref[keep - 1]
     assert np.linalg.norm(P - M) <= (1.01 * np.sqrt(disc) + 1e-11) * np.linalg.norm(M)
+
+
+@pytest.mark.parametrize("n", [150, 400])
+def test_hermitian_back_transformation_matches_unmtr(engine, n):
+    """The blocked complex back-transformation (mode 1) against rocSOLVER
+    zunmtr (mode 2) after the same reduction: equal eigenvectors to rounding
+    (several 64-reflector blocks, split inner products)."""
+    import torch
+    rng = np.random.default_rng(n + 5)
+    Q = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+    lam = np.maximum(1e-9 ** (np.arange(n) / 100.0), 1e-17)
+    A = (Q * lam) @ Q.conj().T
+    A = (A + A.conj().T) / 2
+    out = {}
+    for mode in (1, 2):
+        w, Z = engine.herm_eig(_dev(engine, np.stack([A.real, A.imag])), mode)
+        torch.cuda.synchronize()
+        Zn = Z.cpu().numpy()
+        out[mode] = (w.cpu().numpy(), Zn[0] + 1j * Zn[1])
+    assert np.array_equal(out[1][0], out[2][0])
+    assert np.abs(out[1][1] - out[2][1]).max() <= 1e-12 * np.sqrt(n)
+    Z = out[1][1]
+    assert np.abs(Z.conj() @ Z.T - np.eye(n)).max() <= 1e-12 * np.sqrt(n)
