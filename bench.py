@@ -372,8 +372,9 @@ def main():
                     help="only run device copies of this size (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (DMRG sweeps are sequential over bonds; "
-              "see DESIGN.md for how H_eff would shard); --gpus {} is not supported".format(args.gpus))
+        print("bench.py: runs on one GPU only (DMRG sweeps are sequential over bonds; H_eff would "
+              "shard over MPO channels with one all-reduce of y per matvec); --gpus {} is not "
+              "supported".format(args.gpus))
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy, max(1, args.steps))
