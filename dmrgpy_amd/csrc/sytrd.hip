@@ -20,13 +20,13 @@
 //     from launch 1's partials; p = A22 v with v built on the fly from a;
 //     w = tau (p - V y - W z); partial sums of w^T v.
 //
-// After each panel of TRD_NB columns the trailing matrix is updated with two
-// GEMMs, A22 -= V W^T + W V^T, over the full square (the matrix-vector
-// products read whole rows).  Outputs match dsytrd(uplo = 'L') on the
-// column-major view (the matrix is symmetric, so the row-major buffer is the
-// same matrix): d, e, tau, and reflector j in row j, columns j+2.., ready
-// for rocsolver_dstedc + rocsolver_dormtr.  All sums run in a fixed order
-// (deterministic).
+// After each panel of TRD_NB columns the trailing matrix is updated with one
+// GEMM, A22 -= [V W] [W V]^T = V W^T + W V^T, over the full square (the
+// matrix-vector products read whole rows).  Outputs match dsytrd(uplo = 'L')
+// on the column-major view (the matrix is symmetric, so the row-major buffer
+// is the same matrix): d, e, tau, and reflector j in row j, columns j+2..,
+// ready for rocsolver_dstedc and the back-transformation below.  All sums
+// run in a fixed order (deterministic).
 #include "dmrg_internal.h"
 
 namespace dmrg {
@@ -39,6 +39,10 @@ struct TrdState {
 };
 
 constexpr int TRD_PS = 1 + 2 * TRD_NB;   // partials per workgroup in launch 1
+// The panel's V and W columns are stored per row as [V | W | V] (row pitch
+// TRD_P): one GEMM with inner dimension 2 TRD_NB then applies both halves of
+// the trailing update, A22 -= [V W] [W V]^T, reading and writing A22 once.
+constexpr int TRD_P = 3 * TRD_NB;
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -104,14 +108,14 @@ __global__ __launch_bounds__(256) void trd_col_k(const double* __restrict__ A, i
     const int idx = tid + 256 * u, k = idx % TRD_NB;
     lv[u] = lw[u] = 0.0;
     if (idx < nr * TRD_NB && k < jj) {
-      lv[u] = V[(size_t)r0 * TRD_NB + idx];
-      lw[u] = W[(size_t)r0 * TRD_NB + idx];
+      lv[u] = V[(size_t)(r0 + idx / TRD_NB) * TRD_P + k];
+      lw[u] = W[(size_t)(r0 + idx / TRD_NB) * TRD_P + k];
     }
   }
   double vj = 0.0, wj = 0.0;
   if (tid < jj) {
-    vj = V[(size_t)j * TRD_NB + tid];
-    wj = tid == jj - 1 ? wpre[j] : W[(size_t)j * TRD_NB + tid];
+    vj = V[(size_t)j * TRD_P + tid];
+    wj = tid == jj - 1 ? wpre[j] : W[(size_t)j * TRD_P + tid];
   }
   double arow = 0.0, wpr = 0.0;
   if (tid < nr) {
@@ -142,7 +146,7 @@ __global__ __launch_bounds__(256) void trd_col_k(const double* __restrict__ A, i
   if (jj > 0 && tid < nr) {   // finish column jj-1 of W: w + alpha v
     const double w = wpr + alpha * sV[tid][jj - 1];
     sW[tid][jj - 1] = w;
-    W[(size_t)r * TRD_NB + jj - 1] = w;
+    W[(size_t)r * TRD_P + jj - 1] = w;
   }
   __syncthreads();
   // ---- column j of the panel-start matrix, updated with the panel so far
@@ -209,8 +213,8 @@ __global__ __launch_bounds__(256) void trd_symv_k(double* __restrict__ A, int n,
   }
   double v1 = 0.0, w1 = 0.0;
   if (tid < jj) {
-    v1 = V[(size_t)(j + 1) * TRD_NB + tid];
-    w1 = tid == jj - 1 ? wpre[j + 1] + st->alpha * v1 : W[(size_t)(j + 1) * TRD_NB + tid];
+    v1 = V[(size_t)(j + 1) * TRD_P + tid];
+    w1 = tid == jj - 1 ? wpre[j + 1] + st->alpha * v1 : W[(size_t)(j + 1) * TRD_P + tid];
   }
   double mrow[TRD_ROWS2];
 #pragma unroll
@@ -219,7 +223,7 @@ __global__ __launch_bounds__(256) void trd_symv_k(double* __restrict__ A, int n,
     const double* M = lane < 32 ? V : W;
 #pragma unroll
     for (int i = 0; i < TRD_ROWS2; ++i)
-      if (i < nr) mrow[i] = M[(size_t)(rbase + i) * TRD_NB + (lane & 31)];
+      if (i < nr) mrow[i] = M[(size_t)(rbase + i) * TRD_P + (lane & 31)];
   }
   // ---- the matrix stream: acc = sum_{c >= j+2} A[r][c] a[c], a1 = A[r][j+1]
   double acc[TRD_ROWS2], a1[TRD_ROWS2];
@@ -282,7 +286,8 @@ __global__ __launch_bounds__(256) void trd_symv_k(double* __restrict__ A, int n,
       const double wp = t * q;
       const double vr = r == j + 1 ? 1.0 : scale * avec[r];
       wpre[r] = wp;
-      V[(size_t)r * TRD_NB + jj] = vr;
+      V[(size_t)r * TRD_P + jj] = vr;
+      V[(size_t)r * TRD_P + 2 * TRD_NB + jj] = vr;   // the copy after W
       if (r >= j + 2) A[(size_t)j * n + r] = vr;   // reflector storage (row j)
       pwv = wp * vr;
     }
@@ -304,22 +309,22 @@ __global__ __launch_bounds__(256) void trd_fin_k(int n, int s, int k, const doub
   __shared__ double red4[256];
   const double alpha = -0.5 * st->t * sum_one(part2, g2, red4);
   const int r = s + blockIdx.x * 256 + threadIdx.x;
-  if (r < n) W[(size_t)r * TRD_NB + k] = wpre[r] + alpha * V[(size_t)r * TRD_NB + k];
+  if (r < n) W[(size_t)r * TRD_P + k] = wpre[r] + alpha * V[(size_t)r * TRD_P + k];
 }
 
 // A (n x n, symmetric, row-major = column-major) -> d[n], e[n-1], tau[n-1]
 // and reflectors in place (dsytrd, uplo 'L').
 void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
   DMRG_REQUIRE(n >= 2, "sytrd: order must be at least 2");
-  const size_t vw = (size_t)n * TRD_NB;
+  const size_t vwv = (size_t)n * TRD_P;   // the [V | W | V] panel store
   const int gmax1 = (n + TRD_ROWS1 - 1) / TRD_ROWS1;
   const int gmax2 = (n + TRD_ROWS2 - 1) / TRD_ROWS2;
-  double* buf = (double*)ws_get(c, WS_TRD, sizeof(double) * (2 * vw + 2 * (size_t)n + (size_t)gmax1 * TRD_PS +
+  double* buf = (double*)ws_get(c, WS_TRD, sizeof(double) * (vwv + 2 * (size_t)n + (size_t)gmax1 * TRD_PS +
                                                               (size_t)gmax2 + 8) +
                                                sizeof(TrdState) + 64);
-  double* V = buf;
-  double* W = V + vw;
-  double* wpre = W + vw;
+  double* V = buf;            // row r: V at [0, NB), W at [NB, 2 NB), V again at [2 NB, 3 NB)
+  double* W = V + TRD_NB;
+  double* wpre = V + vwv;
   double* avec = wpre + n;
   double* part1 = avec + n;
   double* part2 = part1 + (size_t)gmax1 * TRD_PS;
@@ -348,13 +353,22 @@ void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
       trd_fin_k<<<(n - s + 255) / 256, 256, 0, c->stream>>>(n, s, nbp - 1, V, W, wpre, part2, g2prev, st);
       DMRG_CHECK_HIP(hipGetLastError());
       const int m = n - s;
-      GemmDesc g = gdesc(m, m, nbp, TRD_NB, TRD_NB, n, REAL, 4);
-      g.transB = true;
-      g.alpha = -1.0;
-      g.beta = 1.0;
       double* C = A + (size_t)s * n + s;
-      gemm_f64(c, g, V + (size_t)s * TRD_NB, W + (size_t)s * TRD_NB, C);
-      gemm_f64(c, g, W + (size_t)s * TRD_NB, V + (size_t)s * TRD_NB, C);
+      const double* Vs = V + (size_t)s * TRD_P;
+      if (nbp == TRD_NB) {   // A22 -= [V W] [W V]^T in one pass over A22
+        GemmDesc g = gdesc(m, m, 2 * TRD_NB, TRD_P, TRD_P, n, REAL, 4);
+        g.transB = true;
+        g.alpha = -1.0;
+        g.beta = 1.0;
+        gemm_f64(c, g, Vs, Vs + TRD_NB, C);
+      } else {   // a short last panel: columns past nbp still hold the previous panel's values
+        GemmDesc g = gdesc(m, m, nbp, TRD_P, TRD_P, n, REAL, 4);
+        g.transB = true;
+        g.alpha = -1.0;
+        g.beta = 1.0;
+        gemm_f64(c, g, Vs, Vs + TRD_NB, C);
+        gemm_f64(c, g, Vs + TRD_NB, Vs, C);
+      }
     }
   }
   DMRG_CHECK_HIP(hipMemcpyAsync(d + (n - 1), A + (size_t)(n - 1) * n + (n - 1), sizeof(double),
