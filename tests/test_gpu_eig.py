@@ -68,7 +68,7 @@ def test_gram_decaying_spectrum(engine, n):
     assert np.abs(w1 - w0).max() <= 1e-13
 
 
-@pytest.mark.parametrize("n", [300, 1100])
+@pytest.mark.parametrize("n", [300, 1100, 2500])
 def test_back_transformation_matches_ormtr(engine, n):
     """The blocked compact-WY back-transformation (mode 1) against rocSOLVER
     dormtr (mode 2) after the same reduction and divide and conquer: equal
