@@ -75,7 +75,7 @@ __device__ __forceinline__ double sum_one(const double* part, int nb, double* re
 }
 
 constexpr int TRD_ROWS1 = 64;   // rows per workgroup in launch 1
-constexpr int TRD_ROWS2 = 4;    // rows per workgroup in launch 2
+constexpr int TRD_ROWS2 = 2;    // rows per workgroup in launch 2 (measured best of 1, 2, 4, 8, 16)
 
 // Both launches issue every global load they need up front (all of them
 // miss the caches: the previous launch wrote the data, possibly from another
