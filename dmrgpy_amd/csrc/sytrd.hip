@@ -874,9 +874,19 @@ __global__ __launch_bounds__(BT_NB) void bt_larft_k(const double* __restrict__ S
   sT[bt_t_idx(k, k)] = stau[k];
   for (int i = k + 1; i < BT_NB; ++i) {
     const double ti = stau[i];
-    double acc = 0.0;
-    for (int l = k; l < i; ++l) acc += sT[bt_t_idx(k, l)] * sS[bt_s_idx(l, i)];
-    sT[bt_t_idx(k, i)] = -ti * acc;
+    // four independent partial sums (fixed order): the chain of dependent
+    // additions, not the LDS reads, bounds this recurrence
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const int tk = bt_t_idx(k, 0), si = bt_s_idx(0, i);   // row k of T and column i of S are contiguous
+    int l = k;
+    for (; l + 3 < i; l += 4) {
+      a0 += sT[tk + l] * sS[si + l];
+      a1 += sT[tk + l + 1] * sS[si + l + 1];
+      a2 += sT[tk + l + 2] * sS[si + l + 2];
+      a3 += sT[tk + l + 3] * sS[si + l + 3];
+    }
+    for (; l < i; ++l) a0 += sT[tk + l] * sS[si + l];
+    sT[bt_t_idx(k, i)] = -ti * ((a0 + a1) + (a2 + a3));
   }
   __syncthreads();
   double* out = TtS + (size_t)b * BT_SPLIT * BT_NB * BT_NB;
@@ -993,13 +1003,23 @@ __global__ __launch_bounds__(BTC_NB) void btc_larft_k(const double* __restrict__
   sTr[btc_t_idx(k, k)] = str[k];
   sTi[btc_t_idx(k, k)] = sti[k];
   for (int i = k + 1; i < BTC_NB; ++i) {
-    double ar = 0.0, ai = 0.0;
-    for (int l = k; l < i; ++l) {
-      const double xr = sTr[btc_t_idx(k, l)], xi = sTi[btc_t_idx(k, l)];
-      const double yr = sSr[btc_s_idx(l, i)], yi = sSi[btc_s_idx(l, i)];
-      ar += xr * yr - xi * yi;
-      ai += xr * yi + xi * yr;
+    double ar0 = 0.0, ai0 = 0.0, ar1 = 0.0, ai1 = 0.0;   // two independent partial sums (fixed order)
+    const int tk = btc_t_idx(k, 0), si = btc_s_idx(0, i);
+    int l = k;
+    for (; l + 1 < i; l += 2) {
+      const double xr = sTr[tk + l], xi = sTi[tk + l], yr = sSr[si + l], yi = sSi[si + l];
+      const double xr1 = sTr[tk + l + 1], xi1 = sTi[tk + l + 1], yr1 = sSr[si + l + 1], yi1 = sSi[si + l + 1];
+      ar0 += xr * yr - xi * yi;
+      ai0 += xr * yi + xi * yr;
+      ar1 += xr1 * yr1 - xi1 * yi1;
+      ai1 += xr1 * yi1 + xi1 * yr1;
     }
+    if (l < i) {
+      const double xr = sTr[tk + l], xi = sTi[tk + l], yr = sSr[si + l], yi = sSi[si + l];
+      ar0 += xr * yr - xi * yi;
+      ai0 += xr * yi + xi * yr;
+    }
+    const double ar = ar0 + ar1, ai = ai0 + ai1;
     const double tr = str[i], ti = sti[i];
     sTr[btc_t_idx(k, i)] = -(tr * ar - ti * ai);
     sTi[btc_t_idx(k, i)] = -(tr * ai + ti * ar);
