@@ -895,6 +895,23 @@ __global__ __launch_bounds__(BT_NB) void bt_larft_k(const double* __restrict__ S
       out[((size_t)s * BT_NB + k) * BT_NB + i] = i <= k ? sT[bt_t_idx(i, k)] : 0.0;
 }
 
+// Y[c][i] = sum_s Ycat[c][s nb + i] (the split inner products of a block, in
+// order), for every plane: rows*nb outputs per plane, plane strides pin/pout
+__global__ void bt_sum_k(const double* __restrict__ Ycat, int rows, int nb, int S, long long pin,
+                         double* __restrict__ Y, long long pout, int planes) {
+  const long long tot = (long long)rows * nb;
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < tot * planes;
+       idx += (long long)gridDim.x * 256) {
+    const int pl = (int)(idx / tot);
+    const long long o = idx % tot;
+    const int c = (int)(o / nb), i = (int)(o % nb);
+    const double* src = Ycat + pl * pin + (size_t)c * S * nb + i;
+    double s = 0.0;
+    for (int q = 0; q < S; ++q) s += src[(size_t)q * nb];
+    Y[pl * pout + o] = s;
+  }
+}
+
 // Zt (keep x n, row-major, the rows eigenvectors of the tridiagonal) <- rows
 // of Q Y, Q from the reflectors in A (row-major n x n as sytrd_lower leaves
 // it) and tau
@@ -904,7 +921,7 @@ void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
   const size_t vrows = (size_t)nblk * BT_NB;
   double* Vall = (double*)ws_get(c, WS_BT_V, sizeof(double) * vrows * n);
   const size_t nS = (size_t)nblk * BT_NB * BT_NB, nT = (size_t)nblk * BT_SPLIT * BT_NB * BT_NB;
-  const size_t nY = (size_t)keep * BT_SPLIT * BT_NB, nY2 = (size_t)keep * BT_NB;
+  const size_t nY = (size_t)keep * (BT_SPLIT + 1) * BT_NB, nY2 = (size_t)keep * BT_NB;   // S partials + their sum
   double* Sm = (double*)ws_get(c, WS_BT_W, sizeof(double) * (nS + nT + nY + nY2));
   double* TtS = Sm + nS;
   double* Ycat = TtS + nT;
@@ -940,8 +957,17 @@ void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
     g1.sA = g1.sB = kc;
     g1.sC = BT_NB;
     gemm_f64(c, g1, Zt + start, Vb + start, Ycat);
-    GemmDesc g2 = gdesc(keep, BT_NB, S * BT_NB, S * BT_NB, BT_NB, BT_NB, REAL, 4);
-    gemm_f64(c, g2, Ycat, TtS + (size_t)b * BT_SPLIT * BT_NB * BT_NB, Y2);
+    const double* Y = Ycat;
+    if (S > 1) {   // sum the split products in place of an inner dimension S*NB in the next GEMM
+      double* Ys = Ycat + (size_t)keep * S * BT_NB;   // room after the S partials
+      const long long tot = (long long)keep * BT_NB;
+      bt_sum_k<<<(int)std::min<long long>((tot + 255) / 256, 4096), 256, 0, c->stream>>>(Ycat, keep, BT_NB, S, 0, Ys,
+                                                                                           0, 1);
+      DMRG_CHECK_HIP(hipGetLastError());
+      Y = Ys;
+    }
+    GemmDesc g2 = gdesc(keep, BT_NB, BT_NB, BT_NB, BT_NB, BT_NB, REAL, 4);
+    gemm_f64(c, g2, Y, TtS + (size_t)b * BT_SPLIT * BT_NB * BT_NB, Y2);
     GemmDesc g3 = gdesc(keep, len, BT_NB, BT_NB, n, n, REAL, 4);
     g3.alpha = -1.0;
     g3.beta = 1.0;
@@ -1044,7 +1070,7 @@ void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
   const long long pV = (long long)vrows * n;
   double* Vall = (double*)ws_get(c, WS_BT_V, sizeof(double) * 2 * (size_t)pV);
   const size_t nS = (size_t)nblk * NB * NB, nT = (size_t)nblk * BT_SPLIT * NB * NB;
-  const size_t nY = (size_t)keep * BT_SPLIT * NB, nY2 = (size_t)keep * NB;
+  const size_t nY = (size_t)keep * (BT_SPLIT + 1) * NB, nY2 = (size_t)keep * NB;   // S partials + their sum
   double* Sm = (double*)ws_get(c, WS_BT_W, sizeof(double) * 2 * (nS + nT + nY + nY2));
   double* TtS = Sm + 2 * nS;
   double* Ycat = TtS + 2 * nT;
@@ -1087,11 +1113,20 @@ void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
     g1.pB = pV;
     g1.pC = (long long)nY;
     gemm_f64(c, g1, Et + start, Vb + start, Ycat);
-    GemmDesc g2 = gdesc(keep, NB, S * NB, S * NB, NB, NB, CPLX, 4);   // Y2 = Y T_b^T
+    const double* Y = Ycat;
+    if (S > 1) {   // sum the split products (both planes)
+      double* Ys = Ycat + (size_t)keep * S * NB;
+      const long long tot = (long long)keep * NB;
+      bt_sum_k<<<(int)std::min<long long>((2 * tot + 255) / 256, 4096), 256, 0, c->stream>>>(
+          Ycat, keep, NB, S, (long long)nY, Ys, (long long)nY, 2);
+      DMRG_CHECK_HIP(hipGetLastError());
+      Y = Ys;
+    }
+    GemmDesc g2 = gdesc(keep, NB, NB, NB, NB, NB, CPLX, 4);   // Y2 = Y T_b^T
     g2.pA = (long long)nY;
     g2.pB = (long long)nT;
     g2.pC = (long long)nY2;
-    gemm_f64(c, g2, Ycat, TtS + (size_t)b * BT_SPLIT * NB * NB, Y2);
+    gemm_f64(c, g2, Y, TtS + (size_t)b * BT_SPLIT * NB * NB, Y2);
     GemmDesc g3 = gdesc(keep, len, NB, NB, n, n, CPLX, 4);   // Et -= Y2 V_b^T
     g3.alpha = -1.0;
     g3.beta = 1.0;
