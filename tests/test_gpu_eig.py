@@ -184,11 +184,13 @@ def test_hermitian_gram_decaying_spectrum(engine, n):
     assert np.abs(w1 - w0).max() <= 1e-13
 
 
-def test_hermitian_special(engine):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_hermitian_special(engine, mode):
     """Block-diagonal matrices (exactly zero sub-columns: the reduction takes
-    the reflection tau = 2 there, since rocSOLVER's zunmtr returns NaN
-    vectors when some reflectors have tau = 0 -- zheevd itself does on these
-    matrices), a real symmetric matrix given as complex, and a diagonal."""
+    the reflection tau = 2 there, since rocSOLVER's zunmtr -- mode 2 --
+    returns NaN vectors when some reflectors have tau = 0, and zheevd itself
+    does on these matrices), a real symmetric matrix given as complex, and a
+    diagonal; with the library's back-transformation (mode 1) and zunmtr."""
     rng = np.random.default_rng(9)
     n = 70
     A = np.zeros((n, n), complex)
@@ -197,10 +199,10 @@ def test_hermitian_special(engine):
         A[b0:b1, b0:b1] = X + X.conj().T
     A[11, 10] = 0.5j   # a lone complex entry right below the diagonal
     A[10, 11] = -0.5j
-    _check_herm(engine, A, 1)
+    _check_herm(engine, A, mode)
     X = rng.standard_normal((40, 40))
-    _check_herm(engine, (X + X.T).astype(complex), 1)
-    _check_herm(engine, np.diag(rng.standard_normal(50)).astype(complex), 1)
+    _check_herm(engine, (X + X.T).astype(complex), mode)
+    _check_herm(engine, np.diag(rng.standard_normal(50)).astype(complex), mode)
 
 
 @pytest.mark.parametrize("other", [1, 2])
