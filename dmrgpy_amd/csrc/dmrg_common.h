@@ -102,11 +102,13 @@ struct Ctx {
   double prof_bytes[5] = {0, 0, 0, 0, 0};   // algorithmic HBM bytes
   long long prof_calls[5] = {0, 0, 0, 0, 0};
   long long matvecs = 0;
-  // dense eigensolver of the Gram route: 1 = own two-launch
-  // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc + own
-  // blocked back-transformation of the kept eigenvectors, 2 = the same with
-  // rocSOLVER's ormtr / unmtr, 0 = rocSOLVER syevd / heevd
-  int eig_mode = 1;
+  // dense eigensolver of the Gram route: 2 (default) = own two-launch
+  // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc +
+  // rocSOLVER ormtr / unmtr on the kept eigenvectors; 1 = the same with the
+  // library's blocked back-transformation (faster, opt-in: intermittent
+  // wrong factors seen in DMRG runs at Gram orders <= 128, cause not found);
+  // 0 = rocSOLVER syevd / heevd
+  int eig_mode = 2;
 };
 
 void* ws_get(Ctx* c, Slot s, size_t bytes);

@@ -386,6 +386,7 @@ static SvdOut svd_full(Ctx* c, int field, const double* M, int m, int n, double 
   const double scale = hmax > 0.0 ? hmax : 1.0;
   for (auto& s : svals) s *= scale;
   DMRG_CHECK_HIP(hipMemcpyAsync(S, svals.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
+  DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));   // svals belongs to the caller: consume it now
   SvdOut o;
   o.keep = truncate_rule(svals, cutoff, maxdim, mindim, &o.discarded, 0.0);
   DMRG_REQUIRE(o.keep <= kcap, "svd: keep exceeds output capacity");

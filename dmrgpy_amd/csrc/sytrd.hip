@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void trc_col_k(const double* __restrict__ A, i
   __shared__ double sar[TRD_ROWS1], sai[TRD_ROWS1];
   __shared__ double redr[256], redi[256];
   const int tid = threadIdx.x;
-  const size_t pA = (size_t)n * n, pV = (size_t)n * TRD_NB;
+  const size_t pA = (size_t)n * n, pV = (size_t)n * TRD_P;
   const int r0 = j + blockIdx.x * TRD_ROWS1;
   const int nr = max(0, min(TRD_ROWS1, n - r0));
   const int r = r0 + tid;
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(256) void trc_col_k(const double* __restrict__ A, i
     const int idx = tid + 256 * u, k = idx % TRD_NB;
     lvr[u] = lvi[u] = lwr[u] = lwi[u] = 0.0;
     if (idx < nr * TRD_NB && k < jj) {
-      const size_t o = (size_t)r0 * TRD_NB + idx;
+      const size_t o = (size_t)(r0 + idx / TRD_NB) * TRD_P + k;
       lvr[u] = V[o];
       lvi[u] = V[o + pV];
       lwr[u] = W[o];
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(256) void trc_col_k(const double* __restrict__ A, i
   }
   double vjr = 0.0, vji = 0.0, wjr = 0.0, wji = 0.0;
   if (tid < jj) {
-    const size_t o = (size_t)j * TRD_NB + tid;
+    const size_t o = (size_t)j * TRD_P + tid;
     vjr = V[o];
     vji = V[o + pV];
     if (tid == jj - 1) {
@@ -499,8 +499,10 @@ __global__ __launch_bounds__(256) void trc_col_k(const double* __restrict__ A, i
     const double wr = wpr + alr * xr - ali * xi, wi = wpi + alr * xi + ali * xr;
     sWr[tid][jj - 1] = wr;
     sWi[tid][jj - 1] = wi;
-    W[(size_t)r * TRD_NB + jj - 1] = wr;
-    W[(size_t)r * TRD_NB + jj - 1 + pV] = wi;
+    W[(size_t)r * TRD_P + jj - 1] = wr;
+    W[(size_t)r * TRD_P + jj - 1 + pV] = wi;
+    W[(size_t)r * TRD_P + 2 * TRD_NB + jj - 1] = wr;   // the copy after V
+    W[(size_t)r * TRD_P + 2 * TRD_NB + jj - 1 + pV] = wi;
   }
   __syncthreads();
   // ---- column j of B, updated with the panel: a -= V conj(W_j)^T + W conj(V_j)^T
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(256) void trc_symv_k(double* __restrict__ A, int n,
   __shared__ double tot[TRC_PS];
   __shared__ double sst[4];   // tau (re, im), scale (re, im)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const size_t pA = (size_t)n * n, pV = (size_t)n * TRD_NB;
+  const size_t pA = (size_t)n * n, pV = (size_t)n * TRD_P;
   const int nk = 1 + 4 * jj;   // <= 125
   const int rbase = j + 1 + blockIdx.x * TRD_ROWS2;
   const int nr = min(TRD_ROWS2, n - rbase);   // >= 1
@@ -581,7 +583,7 @@ __global__ __launch_bounds__(256) void trc_symv_k(double* __restrict__ A, int n,
   }
   double v1r = 0.0, v1i = 0.0, w1r = 0.0, w1i = 0.0;
   if (tid < jj) {
-    const size_t o = (size_t)(j + 1) * TRD_NB + tid;
+    const size_t o = (size_t)(j + 1) * TRD_P + tid;
     v1r = V[o];
     v1i = V[o + pV];
     if (tid == jj - 1) {
@@ -601,7 +603,7 @@ __global__ __launch_bounds__(256) void trc_symv_k(double* __restrict__ A, int n,
 #pragma unroll
     for (int i = 0; i < TRD_ROWS2; ++i)
       if (i < nr) {
-        const size_t o = (size_t)(rbase + i) * TRD_NB + (lane & 31);
+        const size_t o = (size_t)(rbase + i) * TRD_P + (lane & 31);
         mr[i] = M[o];
         mi[i] = M[o + pV];
       }
@@ -703,8 +705,8 @@ __global__ __launch_bounds__(256) void trc_symv_k(double* __restrict__ A, int n,
       }
       wpre[r] = wr;
       wpre[r + n] = wi;
-      V[(size_t)r * TRD_NB + jj] = vr;
-      V[(size_t)r * TRD_NB + jj + pV] = vi;
+      V[(size_t)r * TRD_P + jj] = vr;
+      V[(size_t)r * TRD_P + jj + pV] = vi;
       if (r >= j + 2) {   // reflector storage (row j)
         A[(size_t)j * n + r] = vr;
         A[(size_t)j * n + r + pA] = vi;
@@ -747,13 +749,16 @@ __global__ __launch_bounds__(256) void trc_fin_k(int n, int s, int k, const doub
   const double sr = red4[0] + red4[1] + red4[2] + red4[3], si = red4[4] + red4[5] + red4[6] + red4[7];
   const double tr = st->tr, ti = st->ti;
   const double alr = -0.5 * (tr * sr - ti * si), ali = -0.5 * (tr * si + ti * sr);
-  const size_t pV = (size_t)n * TRD_NB;
+  const size_t pV = (size_t)n * TRD_P;
   const int r = s + blockIdx.x * 256 + threadIdx.x;
   if (r < n) {
-    const size_t o = (size_t)r * TRD_NB + k;
+    const size_t o = (size_t)r * TRD_P + k;
     const double vr = V[o], vi = V[o + pV];
-    W[o] = wpre[r] + alr * vr - ali * vi;
-    W[o + pV] = wpre[r + n] + alr * vi + ali * vr;
+    const double wr = wpre[r] + alr * vr - ali * vi, wi = wpre[r + n] + alr * vi + ali * vr;
+    W[o] = wr;
+    W[o + pV] = wi;
+    W[o + 2 * TRD_NB] = wr;   // the copy after V
+    W[o + 2 * TRD_NB + pV] = wi;
   }
 }
 
@@ -762,15 +767,15 @@ __global__ __launch_bounds__(256) void trc_fin_k(int n, int s, int k, const doub
 // reflectors in place (zhetrd, uplo 'L', on the column-major view).
 void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
   DMRG_REQUIRE(n >= 2, "hetrd: order must be at least 2");
-  const size_t vw = (size_t)n * TRD_NB;
+  const size_t vw = (size_t)n * TRD_P;   // one plane of the [W | V | W] panel store
   const int gmax1 = (n + TRD_ROWS1 - 1) / TRD_ROWS1;
   const int gmax2 = (n + TRD_ROWS2 - 1) / TRD_ROWS2;
-  double* buf = (double*)ws_get(c, WS_TRD, sizeof(double) * (4 * vw + 4 * (size_t)n + (size_t)gmax1 * TRC_PS +
+  double* buf = (double*)ws_get(c, WS_TRD, sizeof(double) * (2 * vw + 4 * (size_t)n + (size_t)gmax1 * TRC_PS +
                                                               2 * (size_t)gmax2 + 8) +
                                                sizeof(TrcState) + 64);
-  double* V = buf;             // planar: re plane, im plane (stride vw)
-  double* W = V + 2 * vw;
-  double* wpre = W + 2 * vw;   // planar, stride n
+  double* W = buf;             // planar (plane stride vw); row r: W at [0, NB), V at [NB, 2 NB), W again
+  double* V = W + TRD_NB;
+  double* wpre = buf + 2 * vw;   // planar, stride n
   double* avec = wpre + 2 * (size_t)n;
   double* part1 = avec + 2 * (size_t)n;
   double* part2 = part1 + (size_t)gmax1 * TRC_PS;
@@ -798,9 +803,11 @@ void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
     if (s < n) {
       trc_fin_k<<<(n - s + 255) / 256, 256, 0, c->stream>>>(n, s, nbp - 1, V, W, wpre, part2, g2prev, st);
       DMRG_CHECK_HIP(hipGetLastError());
-      // buf(r, c) -= conj(W[r]) . V[c] + conj(V[r]) . W[c]   (B22 -= V W^H + W V^H)
+      // buf(r, c) -= conj(W[r]) . V[c] + conj(V[r]) . W[c]   (B22 -= V W^H + W V^H):
+      // one GEMM over [W V] and [V W] for a full panel
       const int m = n - s;
-      GemmDesc g = gdesc(m, m, nbp, TRD_NB, TRD_NB, n, CPLX, 4);
+      const bool full = nbp == TRD_NB;
+      GemmDesc g = gdesc(m, m, full ? 2 * TRD_NB : nbp, TRD_P, TRD_P, n, CPLX, 4);
       g.transB = true;
       g.conjA = true;
       g.alpha = -1.0;
@@ -809,8 +816,9 @@ void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
       g.pB = (long long)vw;
       g.pC = (long long)n * n;
       double* C = A + (size_t)s * n + s;
-      gemm_f64(c, g, W + (size_t)s * TRD_NB, V + (size_t)s * TRD_NB, C);
-      gemm_f64(c, g, V + (size_t)s * TRD_NB, W + (size_t)s * TRD_NB, C);
+      const double* Ws = W + (size_t)s * TRD_P;
+      gemm_f64(c, g, Ws, Ws + TRD_NB, C);
+      if (!full) gemm_f64(c, g, Ws + TRD_NB, Ws, C);   // short last panel: columns past nbp are stale
     }
   }
   DMRG_CHECK_HIP(hipMemcpyAsync(d + (n - 1), A + (size_t)(n - 1) * n + (n - 1), sizeof(double),

@@ -213,12 +213,14 @@ class Engine:
         return U, V, k, disc.value, sv
 
     def set_eigensolver(self, mode):
-        """1: own tridiagonalisation (real or Hermitian) + rocSOLVER stedc +
-        own back-transformation (default); 2: as 1 with rocSOLVER
-        ormtr/unmtr; 0: rocSOLVER syevd/heevd -- for the Gram-route splits."""
+        """2 (default): own tridiagonalisation (real or Hermitian) +
+        rocSOLVER stedc + rocSOLVER ormtr/unmtr; 1: as 2 with the library's
+        blocked back-transformation (faster; opt-in: intermittent wrong
+        factors were seen with it in DMRG runs at small Gram orders);
+        0: rocSOLVER syevd/heevd -- for the Gram-route splits."""
         check(self.lib.dmrg_set_eigensolver(self.ctx, int(mode)), "dmrg_set_eigensolver")
 
-    def herm_eig(self, A, mode=1):
+    def herm_eig(self, A, mode=2):
         """Eigenpairs of a Hermitian device matrix given as planar float64
         [2, n, n] (copied): (w ascending [n], Z [2, n, n] planar with row k
         the eigenvector of w[k])."""
@@ -229,7 +231,7 @@ class Engine:
         check(self.lib.dmrg_herm_eig(self.ctx, int(mode), _ptr(Ac), n, _ptr(w), _ptr(Z)), "dmrg_herm_eig")
         return w, Z
 
-    def sym_eig(self, A, mode=1):
+    def sym_eig(self, A, mode=2):
         """Eigenpairs of a symmetric float64 device matrix [n, n] (copied):
         (w ascending [n], Z [n, n] with row k the eigenvector of w[k])."""
         n = A.shape[-1]

@@ -150,14 +150,15 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
                           double* resid);
 
 /* Dense eigensolver of the Gram route (the np.linalg.eigh call of
- * _svd_truncated, src/dmrgpy/pyitensor/svd.py:134): mode 1 (the default)
+ * _svd_truncated, src/dmrgpy/pyitensor/svd.py:134): mode 2 (the default)
  * reduces the real symmetric or complex Hermitian Gram matrix to a real
  * tridiagonal with this library's two-launch-per-column Householder
  * kernels, runs rocSOLVER's divide and conquer on the tridiagonal and
- * back-transforms only the eigenvectors the truncation keeps with this
- * library's blocked compact-WY GEMMs; mode 2 is mode 1 with rocSOLVER's
- * dormtr / zunmtr for the back-transformation; mode 0 calls rocSOLVER
- * dsyevd / zheevd. */
+ * back-transforms only the eigenvectors the truncation keeps with
+ * rocSOLVER's dormtr / zunmtr; mode 1 does the back-transformation with this
+ * library's blocked compact-WY GEMMs instead (faster; opt-in: intermittent
+ * wrong factors were observed with it in DMRG runs at Gram orders <= 128,
+ * cause not found); mode 0 calls rocSOLVER dsyevd / zheevd. */
 int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode);
 
 /* All eigenpairs of the real symmetric n x n matrix A (device, row-major;
@@ -169,9 +170,9 @@ int dmrg_sym_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z
 /* All eigenpairs of the Hermitian n x n matrix A (device, row-major, planar
  * complex: real plane then imaginary plane, plane stride n*n; overwritten):
  * w ascending, row k of Z (planar, same layout) the eigenvector of w[k].
- * mode 1: this library's reduction to a real tridiagonal (zhetrd
- * conventions), rocSOLVER dstedc and this library's back-transformation;
- * mode 2: the same with rocSOLVER zunmtr; mode 0: rocSOLVER zheevd.  The
+ * mode 2 (the split's default): this library's reduction to a real
+ * tridiagonal (zhetrd conventions), rocSOLVER dstedc and zunmtr; mode 1: the
+ * same with this library's back-transformation; mode 0: rocSOLVER zheevd.  The
  * eigensolver of the complex dmrg_svd_split exposed for its parity tests;
  * replaces np.linalg.eigh on the complex Gram matrix (svd.py:134). */
 int dmrg_herm_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z);
