@@ -211,10 +211,13 @@ __global__ __launch_bounds__(256) void trd_symv_k(double* __restrict__ A, int n,
     a0 = avec[j + 1];
     aj = avec[j];
   }
+  // row j+1 of V and W (column jj-1 of W was finished by launch 1, which
+  // covers rows >= j; wpre[j+1] must not be read here: workgroup 0 of this
+  // launch overwrites it with the new column)
   double v1 = 0.0, w1 = 0.0;
   if (tid < jj) {
     v1 = V[(size_t)(j + 1) * TRD_P + tid];
-    w1 = tid == jj - 1 ? wpre[j + 1] + st->alpha * v1 : W[(size_t)(j + 1) * TRD_P + tid];
+    w1 = W[(size_t)(j + 1) * TRD_P + tid];
   }
   double mrow[TRD_ROWS2];
 #pragma unroll
@@ -581,19 +584,16 @@ __global__ __launch_bounds__(256) void trc_symv_k(double* __restrict__ A, int n,
     a0i = avec[j + 1 + n];
     aj = avec[j];
   }
+  // row j+1 of V and W (column jj-1 of W was finished by launch 1, which
+  // covers rows >= j; wpre[j+1] must not be read here: workgroup 0 of this
+  // launch overwrites it with the new column)
   double v1r = 0.0, v1i = 0.0, w1r = 0.0, w1i = 0.0;
   if (tid < jj) {
     const size_t o = (size_t)(j + 1) * TRD_P + tid;
     v1r = V[o];
     v1i = V[o + pV];
-    if (tid == jj - 1) {
-      const double alr = st->alr, ali = st->ali;
-      w1r = wpre[j + 1] + alr * v1r - ali * v1i;
-      w1i = wpre[j + 1 + n] + alr * v1i + ali * v1r;
-    } else {
-      w1r = W[o];
-      w1i = W[o + pV];
-    }
+    w1r = W[o];
+    w1i = W[o + pV];
   }
   double mr[TRD_ROWS2], mi[TRD_ROWS2];
 #pragma unroll
