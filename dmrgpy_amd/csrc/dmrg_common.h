@@ -105,9 +105,8 @@ struct Ctx {
   // dense eigensolver of the Gram route: 2 (default) = own two-launch
   // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc +
   // rocSOLVER ormtr / unmtr on the kept eigenvectors; 1 = the same with the
-  // library's blocked back-transformation (faster, opt-in: intermittent
-  // wrong factors seen in DMRG runs at Gram orders <= 128, cause not found);
-  // 0 = rocSOLVER syevd / heevd
+  // library's blocked back-transformation (faster; opt-in until its sweep
+  // timings are re-taken); 0 = rocSOLVER syevd / heevd
   int eig_mode = 2;
 };
 

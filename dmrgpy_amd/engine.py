@@ -215,9 +215,9 @@ class Engine:
     def set_eigensolver(self, mode):
         """2 (default): own tridiagonalisation (real or Hermitian) +
         rocSOLVER stedc + rocSOLVER ormtr/unmtr; 1: as 2 with the library's
-        blocked back-transformation (faster; opt-in: intermittent wrong
-        factors were seen with it in DMRG runs at small Gram orders);
-        0: rocSOLVER syevd/heevd -- for the Gram-route splits."""
+        blocked back-transformation (faster; opt-in until its sweep
+        timings are re-taken); 0: rocSOLVER syevd/heevd -- for the
+        Gram-route splits."""
         check(self.lib.dmrg_set_eigensolver(self.ctx, int(mode)), "dmrg_set_eigensolver")
 
     def herm_eig(self, A, mode=2):

