@@ -424,7 +424,7 @@ def main():
     if not args.no_syevd_check:
         eng.set_eigensolver(0)
         ressy = run_sweeps(ch, start.copy(), args.steps, args.precision, D)
-        eng.set_eigensolver(2)
+        eng.set_eigensolver(1)   # back to the default
         log("timed {} with rocSOLVER dsyevd: {}".format(args.precision, ressy))
     resfull = None
     if args.svd == "guided" and not args.no_full_svd_check:

@@ -102,12 +102,12 @@ struct Ctx {
   double prof_bytes[5] = {0, 0, 0, 0, 0};   // algorithmic HBM bytes
   long long prof_calls[5] = {0, 0, 0, 0, 0};
   long long matvecs = 0;
-  // dense eigensolver of the Gram route: 2 (default) = own two-launch
-  // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc +
-  // rocSOLVER ormtr / unmtr on the kept eigenvectors; 1 = the same with the
-  // library's blocked back-transformation (faster; opt-in until its sweep
-  // timings are re-taken); 0 = rocSOLVER syevd / heevd
-  int eig_mode = 2;
+  // dense eigensolver of the Gram route: 1 (default) = own two-launch
+  // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc + the
+  // library's blocked back-transformation of the kept eigenvectors; 2 = the
+  // same with rocSOLVER ormtr / unmtr for the back-transformation;
+  // 0 = rocSOLVER syevd / heevd
+  int eig_mode = 1;
 };
 
 void* ws_get(Ctx* c, Slot s, size_t bytes);

@@ -213,11 +213,11 @@ class Engine:
         return U, V, k, disc.value, sv
 
     def set_eigensolver(self, mode):
-        """2 (default): own tridiagonalisation (real or Hermitian) +
-        rocSOLVER stedc + rocSOLVER ormtr/unmtr; 1: as 2 with the library's
-        blocked back-transformation (faster; opt-in until its sweep
-        timings are re-taken); 0: rocSOLVER syevd/heevd -- for the
-        Gram-route splits."""
+        """1 (default): own tridiagonalisation (real or Hermitian) +
+        rocSOLVER stedc + the library's blocked back-transformation of the
+        kept eigenvectors; 2: as 1 with rocSOLVER ormtr/unmtr for the
+        back-transformation; 0: rocSOLVER syevd/heevd -- for the Gram-route
+        splits."""
         check(self.lib.dmrg_set_eigensolver(self.ctx, int(mode)), "dmrg_set_eigensolver")
 
     def herm_eig(self, A, mode=2):

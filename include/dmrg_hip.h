@@ -150,14 +150,14 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
                           double* resid);
 
 /* Dense eigensolver of the Gram route (the np.linalg.eigh call of
- * _svd_truncated, src/dmrgpy/pyitensor/svd.py:134): mode 2 (the default)
+ * _svd_truncated, src/dmrgpy/pyitensor/svd.py:134): mode 1 (the default)
  * reduces the real symmetric or complex Hermitian Gram matrix to a real
  * tridiagonal with this library's two-launch-per-column Householder
  * kernels, runs rocSOLVER's divide and conquer on the tridiagonal and
- * back-transforms only the eigenvectors the truncation keeps with
- * rocSOLVER's dormtr / zunmtr; mode 1 does the back-transformation with this
- * library's blocked compact-WY GEMMs instead (faster; opt-in until its
- * sweep timings are re-taken); mode 0 calls rocSOLVER dsyevd / zheevd. */
+ * back-transforms only the eigenvectors the truncation keeps with this
+ * library's blocked compact-WY GEMMs; mode 2 does the back-transformation
+ * with rocSOLVER's dormtr / zunmtr instead; mode 0 calls rocSOLVER
+ * dsyevd / zheevd. */
 int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode);
 
 /* All eigenpairs of the real symmetric n x n matrix A (device, row-major;
