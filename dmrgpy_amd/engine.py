@@ -220,7 +220,7 @@ class Engine:
         splits."""
         check(self.lib.dmrg_set_eigensolver(self.ctx, int(mode)), "dmrg_set_eigensolver")
 
-    def herm_eig(self, A, mode=2):
+    def herm_eig(self, A, mode=1):
         """Eigenpairs of a Hermitian device matrix given as planar float64
         [2, n, n] (copied): (w ascending [n], Z [2, n, n] planar with row k
         the eigenvector of w[k])."""
@@ -231,7 +231,7 @@ class Engine:
         check(self.lib.dmrg_herm_eig(self.ctx, int(mode), _ptr(Ac), n, _ptr(w), _ptr(Z)), "dmrg_herm_eig")
         return w, Z
 
-    def sym_eig(self, A, mode=2):
+    def sym_eig(self, A, mode=1):
         """Eigenpairs of a symmetric float64 device matrix [n, n] (copied):
         (w ascending [n], Z [n, n] with row k the eigenvector of w[k])."""
         n = A.shape[-1]

@@ -169,9 +169,10 @@ int dmrg_sym_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z
 /* All eigenpairs of the Hermitian n x n matrix A (device, row-major, planar
  * complex: real plane then imaginary plane, plane stride n*n; overwritten):
  * w ascending, row k of Z (planar, same layout) the eigenvector of w[k].
- * mode 2 (the split's default): this library's reduction to a real
- * tridiagonal (zhetrd conventions), rocSOLVER dstedc and zunmtr; mode 1: the
- * same with this library's back-transformation; mode 0: rocSOLVER zheevd.  The
+ * mode 1 (the split's default): this library's reduction to a real
+ * tridiagonal (zhetrd conventions), rocSOLVER dstedc and this library's
+ * blocked back-transformation; mode 2: the same with rocSOLVER zunmtr for
+ * the back-transformation; mode 0: rocSOLVER zheevd.  The
  * eigensolver of the complex dmrg_svd_split exposed for its parity tests;
  * replaces np.linalg.eigh on the complex Gram matrix (svd.py:134). */
 int dmrg_herm_eig(dmrg_ctx* ctx, int mode, double* A, int n, double* w, double* Z);

@@ -18,8 +18,9 @@ is arithmetically identical to the reference's link-free boundary tensors):
 Modules and what pins them (each function cites the reference line it
 follows):
 
-* ``sites``      operator tables        -- pinned by commutation relations and
-  the reference's own table entries in tests/test_oracle_sites.py
+* ``sites``      operator tables        -- pinned by the spin algebra
+  (S^2 = S(S+1)) and against the product's own tables in
+  tests/test_mpo_host.py (test_spin_algebra, test_site_tables_match_oracle)
 * ``mpo``        terms -> MPO, dense H  -- pinned by dense Kronecker H vs ED
 * ``linalg``     truncation rule, Gram SVD, Lanczos, Krylov expm -- pinned by
   brute force / numpy.linalg / scipy.linalg.expm on small inputs
@@ -28,7 +29,8 @@ follows):
   golden energies written from the reference (tests/golden/)
 * ``tdvp``       TDVP step              -- pinned by dense expm at N<=8 and the
   golden correlators from the reference
-* ``ed``         exact diagonalisation  -- pinned by closed forms (dimer, N=3,
-  free fermions)
-* ``free_fermion`` XX-chain closed forms
+* ``ed``         exact diagonalisation and the XX-chain free-fermion closed
+  forms (``free_fermion_xx``, ``free_fermion_xx_szsz``) -- pinned by the
+  reference's ED values (dimer, N=3, N=10, TFIM N=4) and by each other at
+  small N (tests/test_oracle.py::test_ed_pins, test_xx_free_fermions)
 """

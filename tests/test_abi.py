@@ -61,3 +61,32 @@ def test_version_and_error_are_callable_without_gpu():
     lib = _lib.load()
     assert lib.dmrg_version() >= 100
     assert lib.dmrg_last_error() is not None
+
+
+def _header_struct_fields(cname):
+    text = open(os.path.join(ROOT, "include", "dmrg_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct " + cname + r"\s*\{(.*?)\}\s*" + cname + ";", text, flags=re.S).group(1)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        decl = re.sub(r"^(const\s+)?[a-z_ ]+?\s*\**\s*(?=[A-Za-z_][A-Za-z0-9_]*\s*(,|$))", "", decl)
+        names += [n.strip().lstrip("*").strip() for n in decl.split(",")]
+    return names
+
+
+def test_documented_heff_desc_matches_binding_and_header():
+    """The HeffDesc snippet in INTEGRATION.md is what a maintainer copies:
+    its fields, in order, must be those of _lib.HeffDesc and of struct
+    dmrg_heff_desc in include/dmrg_hip.h (a short struct would let the
+    library read nproj / proj past its end)."""
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = re.search(r"class HeffDesc\(ctypes\.Structure\):.*?_fields_ = \[(.*?)\]\n", doc, flags=re.S).group(1)
+    documented = re.findall(r'\("([A-Za-z0-9_]+)", ctypes\.(c_[a-z_]+)\)', block)
+    binding = [(n, t.__name__) for n, t in _lib.HeffDesc._fields_]
+    assert [n for n, _ in documented] == [n for n, _ in binding]
+    assert [t for _, t in documented] == [t for _, t in binding]
+    assert [n for n, _ in binding] == _header_struct_fields("dmrg_heff_desc")
+    assert [n for n, _ in _lib.EnvDesc._fields_] == _header_struct_fields("dmrg_env_desc")

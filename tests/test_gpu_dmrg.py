@@ -53,7 +53,8 @@ def test_golden_sweeps_fp64(case):
     # as the reference does, which amplifies rounding differences (the
     # oracle itself differs from the reference there by up to 4e-11/site).
     # Every later sweep must agree to rounding.
-    assert err[0] < 1e-9, err
+    print("case", case["name"], "first-sweep |dE|/N", err[0], "later sweeps max", err[1:].max(), flush=True)
+    assert err[0] < 1e-10, err
     assert err[1:].max() < 1e-11, err
     for name, val in case.get("vev", []):
         fs = {"SzSz_5_6": [(1.0, [("Sz", 5), ("Sz", 6)])], "SxSx_1_2": [(1.0, [("Sx", 1), ("Sx", 2)])],
@@ -95,20 +96,62 @@ def test_tdvp_golden():
     assert np.abs(np.array(corr2) - ref2).max() < 1e-10
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
-def test_reduced_precision_vs_fp64(precision):
+def _heisenberg32_energies(schedules, seed=1):
     from dmrgpy_amd.multioperator import heisenberg_terms
     n, D = 32, 64
     terms = heisenberg_terms(n)
-    energies = {}
-    for prec in ("fp64", precision):
+    out = {}
+    for name, sched in schedules.items():
         ch = make_session([2] * n, terms, D, precision="fp64", field="real")
-        ch.set_sweep_params(D, 6, 1e-12, 0.0)
-        if prec != "fp64":
-            ch.precision_schedule = [prec] * 5 + ["fp64"] if prec == "bf16x3" else [prec] * 6
-        np.random.seed(1)
-        energies[prec] = ch.gs_energy()
-    assert abs(energies[precision] - energies["fp64"]) / n < 1e-9, energies
+        ch.set_sweep_params(D, len(sched), 1e-12, 0.0)
+        ch.precision_schedule = sched
+        np.random.seed(seed)
+        out[name] = ch.gs_energy()
+    return n, out
+
+
+def test_fp32_vs_fp64():
+    """Six FP32-contraction sweeps (FP64 Lanczos/SVD), final FP64 Rayleigh
+    quotient, against six FP64 sweeps from the same seeded MPS: the
+    north-star bound |dE/N| < 1e-9."""
+    n, e = _heisenberg32_energies({"fp64": ["fp64"] * 6, "fp32": ["fp32"] * 6})
+    err = abs(e["fp32"] - e["fp64"]) / n
+    print("fp32 vs fp64 |dE|/N", err, flush=True)
+    assert err < 1e-9, e
+
+
+# Measured accuracy of sweeps that stay in split-BF16x3 to the end (FP64
+# Lanczos / SVD and the final FP64 Rayleigh quotient): the operator's ~1e-5
+# relative rounding stops the local solves at the 1e-4 Ritz-residual floor, so
+# this path does NOT reach the 1e-9 per-site target on its own; the bound
+# below is what it reaches on this chain (N=32, D=64), with a margin.
+BF16X3_MEASURED_BOUND = 2e-6
+
+
+def test_bf16x3_sweeps_accuracy():
+    n, e = _heisenberg32_energies({"fp64": ["fp64"] * 6, "bf16x3": ["bf16x3"] * 6})
+    err = abs(e["bf16x3"] - e["fp64"]) / n
+    print("bf16x3 (all sweeps) vs fp64 |dE|/N", err, flush=True)
+    assert e["bf16x3"] > e["fp64"] - 1e-9 * n      # variational: never below the FP64 energy
+    assert err < BF16X3_MEASURED_BOUND, e
+
+
+def test_bf16x3_then_fp32_schedule():
+    """BF16x3 for the early sweeps, FP32 for the last two (the mixed
+    schedule): reaches the 1e-9 per-site target without an FP64 sweep."""
+    n, e = _heisenberg32_energies({"fp64": ["fp64"] * 6, "mixed": ["bf16x3"] * 4 + ["fp32"] * 2})
+    err = abs(e["mixed"] - e["fp64"]) / n
+    print("bf16x3x4 + fp32x2 vs fp64 |dE|/N", err, flush=True)
+    assert err < 1e-9, e
+
+
+def test_bf16x3_then_fp64_schedule():
+    """BF16x3 sweeps finished by one FP64 sweep (masks BF16x3 rounding; kept
+    as a separately named check of the schedule mechanics)."""
+    n, e = _heisenberg32_energies({"fp64": ["fp64"] * 6, "mixed": ["bf16x3"] * 5 + ["fp64"]})
+    err = abs(e["mixed"] - e["fp64"]) / n
+    print("bf16x3x5 + fp64 vs fp64 |dE|/N", err, flush=True)
+    assert err < 1e-9, e
 
 
 def test_front_end_spin_chain():
