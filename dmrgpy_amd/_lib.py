@@ -45,6 +45,7 @@ SIGNATURES = {
     "dmrg_ctx_destroy": (c_int, [c_void_p]),
     "dmrg_sync": (c_int, [c_void_p]),
     "dmrg_heff_apply": (c_int, [c_void_p, ctypes.POINTER(HeffDesc), c_void_p, c_void_p]),
+    "dmrg_heff_apply_noalloc": (c_int, [c_void_p, ctypes.POINTER(HeffDesc), c_void_p, c_void_p]),
     "dmrg_rayleigh": (c_int, [c_void_p, ctypes.POINTER(HeffDesc), c_void_p, P_double]),
     "dmrg_env_update": (c_int, [c_void_p, ctypes.POINTER(EnvDesc)]),
     "dmrg_lanczos_ground": (c_int, [c_void_p, ctypes.POINTER(HeffDesc), c_void_p, c_int, c_double,

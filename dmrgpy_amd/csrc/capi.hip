@@ -3,12 +3,55 @@
 // message is kept per thread for dmrg_last_error().
 #include <complex>
 #include <cstring>
+#include <mutex>
 
 #include "dmrg_internal.h"
 
 struct dmrg_ctx {
   dmrg::Ctx c;
 };
+
+// Workspace of the allocation-free entry point (dmrg_heff_apply_noalloc):
+// one static device array per GPU, held until the process ends.  Host code
+// never passes `scratch` itself to a kernel (in host code the name is a
+// host-side stand-in): dmrg_ctx_create maps each device's copy once with
+// hipGetSymbolAddress and the library uses that device address.
+static constexpr size_t kScratchBytes = size_t(64) << 20;   // 64 MiB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current).
+unsigned char* scratch_address(int device) {
+  DMRG_REQUIRE(device >= 0 && device < kMaxDevices, "ctx_create: device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* ptr = nullptr;
+    DMRG_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    DMRG_REQUIRE(ptr != nullptr, "ctx_create: hipGetSymbolAddress returned a null scratch address");
+    g_scratch_addr[device] = (unsigned char*)ptr;
+  }
+  return g_scratch_addr[device];
+}
+
+// Switches a context to carving its workspace out of the scratch array for
+// one allocation-free call, and back on every exit path.
+struct NoallocScope {
+  dmrg::Ctx* c;
+  explicit NoallocScope(dmrg::Ctx* c_) : c(c_) {
+    c->noalloc = true;
+    c->scratch_used = 0;
+    for (int i = 0; i < dmrg::WS_NSLOTS; ++i) {
+      c->scratch_slot[i] = nullptr;
+      c->scratch_slot_bytes[i] = 0;
+    }
+  }
+  ~NoallocScope() { c->noalloc = false; }
+};
+}  // namespace
 
 namespace dmrg {
 
@@ -28,7 +71,21 @@ int svd_split_guided_entry(Ctx* c, int field, const double* M, int m, int n, dou
 void mpo_grad(Ctx* c, int field, int prec, const void* L, const double* bra64, const double* ket64, const void* R,
               int Dl, int Dr, int Dlb, int Drb, int d, int wl, int wr, int ldL, int ldR, double* G_host);
 
-void* ws_get(Ctx* c, Slot s, size_t bytes) { return c->ws[s].get(bytes, c->stream); }
+void* ws_get(Ctx* c, Slot s, size_t bytes) {
+  if (!c->noalloc) return c->ws[s].get(bytes, c->stream);
+  // allocation-free call: one 256-byte-aligned region of the scratch array
+  // per slot, handed out in order
+  if (c->scratch_slot[s] && c->scratch_slot_bytes[s] >= bytes) return c->scratch_slot[s];
+  const size_t need = (bytes + 255) / 256 * 256;
+  if (c->scratch_used + need > kScratchBytes)
+    throw Error("allocation-free call: workspace exceeds the " + std::to_string(kScratchBytes) +
+                "-byte scratch array (needs at least " + std::to_string(c->scratch_used + need) + " bytes)");
+  void* p = c->scratch + c->scratch_used;
+  c->scratch_used += need;
+  c->scratch_slot[s] = p;
+  c->scratch_slot_bytes[s] = need;
+  return p;
+}
 
 static hipEvent_t pool_event(Ctx* c) {
   if (!c->event_pool.empty()) {
@@ -65,7 +122,7 @@ static void prof_drain(Ctx* c, bool all) {
 
 ProfScope::ProfScope(Ctx* c_, int kind_, double flops_, double bytes_)
     : c(c_), kind(kind_), flops(flops_), bytes(bytes_) {
-  if (!c->prof_on) return;
+  if (!c->prof_on || c->noalloc) return;   // allocation-free calls create no events
   a = pool_event(c);
   b = pool_event(c);
   DMRG_CHECK_HIP(hipEventRecord(a, c->stream));
@@ -128,6 +185,14 @@ int dmrg_ctx_create(int device, void* stream, dmrg_ctx** out) {
   rocblas_set_stream(h->c.blas, h->c.stream);
   DMRG_CHECK_HIP(hipHostMalloc((void**)&h->c.host_scalars, 64 * sizeof(double), hipHostMallocDefault));
   std::memset(h->c.host_scalars, 0, 64 * sizeof(double));
+  try {
+    h->c.scratch = scratch_address(device);   // device current (hipSetDevice above)
+  } catch (...) {
+    (void)hipHostFree(h->c.host_scalars);
+    rocblas_destroy_handle(h->c.blas);
+    delete h;
+    throw;
+  }
   *out = h;
   API_END
 }
@@ -160,6 +225,22 @@ int dmrg_heff_apply(dmrg_ctx* ctx, const dmrg_heff_desc* op, const double* x, do
   DMRG_REQUIRE(ctx && op && x && y, "heff_apply: null argument");
   dmrg::HeffOp h = dmrg::heff_prepare(&ctx->c, *op);
   dmrg::heff_apply(&ctx->c, h, x, y);
+  API_END
+}
+
+int dmrg_heff_apply_noalloc(dmrg_ctx* ctx, const dmrg_heff_desc* op, const double* x, double* y) {
+  API_BEGIN
+  DMRG_REQUIRE(ctx && op && x && y, "heff_apply_noalloc: null argument");
+  dmrg::Ctx* c = &ctx->c;
+  DMRG_REQUIRE(c->scratch != nullptr, "heff_apply_noalloc: no scratch array mapped for this context");
+  const size_t need = dmrg::heff_scratch_bytes(*op);
+  if (need > kScratchBytes)
+    throw Error("heff_apply_noalloc: this operator needs up to " + std::to_string(need) +
+                " bytes of workspace, more than the " + std::to_string(kScratchBytes) +
+                "-byte scratch array; use dmrg_heff_apply");
+  NoallocScope guard(c);
+  dmrg::HeffOp h = dmrg::heff_prepare(c, *op);
+  dmrg::heff_apply(c, h, x, y);
   API_END
 }
 

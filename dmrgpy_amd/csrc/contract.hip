@@ -17,6 +17,8 @@
 #include "dmrg_common.h"
 #include "dmrg_internal.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace dmrg {
@@ -577,9 +579,7 @@ static int pad4(long long v) { return (int)((v + 3) / 4 * 4); }
 static int pad8(long long v) { return (int)((v + 7) / 8 * 8); }
 static int t2_pitch(int wr, int Dr) { return pad8((long long)wr * Dr); }
 
-HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
-  HeffOp h;
-  h.d = d;
+static void heff_check(const dmrg_heff_desc& d) {
   DMRG_REQUIRE(d.nsite == 1 || d.nsite == 2, "heff: nsite must be 1 or 2");
   DMRG_REQUIRE(d.Dl > 0 && d.Dr > 0 && d.d1 > 0 && d.wl > 0 && d.wr > 0, "heff: bad dims");
   DMRG_REQUIRE(d.field == REAL || d.field == CPLX, "heff: bad field");
@@ -589,10 +589,51 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
                "heff: environment row pitch smaller than the dimension");
   DMRG_REQUIRE(d.nproj >= 0 && d.nproj <= 64 && (d.nproj == 0 || d.proj != nullptr),
                "heff: bad penalty projectors");
+  if (d.nsite == 2) DMRG_REQUIRE(d.d2 > 0 && d.wm > 0, "heff: bad two-site dims");
+}
+
+size_t heff_scratch_bytes(const dmrg_heff_desc& d) {
+  heff_check(d);
+  // every request is rounded up to 256 bytes, plus 256 of slack per slot
+  auto up = [](double b) { return std::ceil(b / 256.0) * 256.0 + 256.0; };
+  const double P = d.nsite == 2 ? (double)d.d1 * d.d2 : (double)d.d1;
+  const double Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr;
+  const double np = d.field == CPLX ? 2.0 : 1.0, es = (double)prec_size(d.prec);
+  const double n = Dl * P * Dr, nT = wl * n, ldT2 = t2_pitch(d.wr, d.Dr), nT2 = Dl * P * ldT2;
+  // small operator (upload_smallop): padded dense W, offset tables and the
+  // compressed rows at their largest (every entry nonzero)
+  const double Kin = wl * P, Kout = P * wr, KoutPad = std::ceil(Kout / OCH_PAD) * OCH_PAD;
+  double total = up(up(KoutPad * Kin * (d.w_complex ? 2.0 : 1.0) * es) + up(4.0 * (Kin + 2 * Kout + 6 + Kout * Kin)) +
+                    up(Kout * Kin * es) + 256.0);
+  // generic path: theta in the contraction precision (pitched), y in the
+  // contraction precision, T1, T2
+  const double Pdr = P * Dr, ldx = std::fmod(Pdr, 4.0) == 0.0 ? Pdr : (double)pad4((long long)Pdr);
+  double gen = up(nT * np * es) + up(nT2 * np * es);
+  if (d.prec != P64 || ldx != Pdr) gen += up(Dl * ldx * np * es);
+  if (d.prec != P64 && !(d.prec == P32 && d.field == REAL)) gen += up(n * np * es);
+  double path = gen;
+  if (d.prec == PB3 && d.field == REAL) {
+    // pre-split operands: L and R^T (hi, lo), theta^T (hi, lo), T1 FP32,
+    // T2 (hi, lo), y FP32
+    const double ldLb = pad8(d.Dl), ldRb = pad8((long long)d.wr * d.Dr);
+    const double b3 = up(4.0 * wl * Dl * ldLb) + up(4.0 * Dr * ldRb) + up(4.0 * P * Dr * pad8(d.Dl)) + up(4.0 * nT) +
+                      up(4.0 * Dl * P * ldT2) + up(4.0 * n);
+    path = std::max(path, b3);
+  }
+  total += path;
+  // penalty projectors: coefficients and the partial sums of the block dot
+  // products (at most 1024 partial blocks per row)
+  if (d.nproj > 0) total += up(16.0 * d.nproj) + up(8.0 * np * d.nproj * 1024.0);
+  return (size_t)total;
+}
+
+HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
+  HeffOp h;
+  h.d = d;
+  heff_check(d);
   const int Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr;
   const bool wc = d.w_complex != 0;
   if (d.nsite == 2) {
-    DMRG_REQUIRE(d.d2 > 0 && d.wm > 0, "heff: bad two-site dims");
     const int d1 = d.d1, d2 = d.d2, wm = d.wm, P = d1 * d2;
     const int Kin = wl * P, Kout = P * wr;
     std::vector<double> wre((size_t)Kout * Kin, 0.0), wim(wc ? (size_t)Kout * Kin : 0, 0.0);

@@ -108,6 +108,15 @@ struct Ctx {
   // same with rocSOLVER ormtr / unmtr for the back-transformation;
   // 0 = rocSOLVER syevd / heevd
   int eig_mode = 1;
+  // allocation-free calls (dmrg_heff_apply_noalloc): while `noalloc` is set,
+  // ws_get carves the workspace out of this device's static scratch array
+  // (device address `scratch`, mapped at context creation) instead of the
+  // growing DevBuf slots; nothing is allocated or freed then
+  unsigned char* scratch = nullptr;
+  bool noalloc = false;
+  size_t scratch_used = 0;
+  void* scratch_slot[WS_NSLOTS] = {};
+  size_t scratch_slot_bytes[WS_NSLOTS] = {};
 };
 
 void* ws_get(Ctx* c, Slot s, size_t bytes);

@@ -77,6 +77,10 @@ void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
 GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag);
 
 double heff_flops(const dmrg_heff_desc& d);
+// Upper bound of the workspace bytes heff_prepare + heff_apply request for
+// the descriptor (each slot rounded up to 256 bytes), for the allocation-free
+// entry point's size check
+size_t heff_scratch_bytes(const dmrg_heff_desc& d);
 HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d);
 void heff_apply(Ctx* c, const HeffOp& h, const double* x, double* y);
 void env_update(Ctx* c, const dmrg_env_desc& e);

@@ -132,6 +132,13 @@ class Engine:
     def heff_apply(self, desc, x, y):
         check(self.lib.dmrg_heff_apply(self.ctx, ctypes.byref(desc), _ptr(x), _ptr(y)), "dmrg_heff_apply")
 
+    def heff_apply_noalloc(self, desc, x, y):
+        """heff_apply through the allocation-free entry point (workspace in
+        the library's static scratch array; raises for operators too large
+        for it)."""
+        check(self.lib.dmrg_heff_apply_noalloc(self.ctx, ctypes.byref(desc), _ptr(x), _ptr(y)),
+              "dmrg_heff_apply_noalloc")
+
     def rayleigh(self, desc, x):
         v = ctypes.c_double()
         check(self.lib.dmrg_rayleigh(self.ctx, ctypes.byref(desc), _ptr(x), ctypes.byref(v)), "dmrg_rayleigh")

@@ -85,6 +85,22 @@ int dmrg_sync(dmrg_ctx* ctx);
  * Dl*d1*d2*Dr (nsite 2) or Dl*d1*Dr (nsite 1) per plane. */
 int dmrg_heff_apply(dmrg_ctx* ctx, const dmrg_heff_desc* op, const double* x, double* y);
 
+/* dmrg_heff_apply that allocates nothing: no counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same arguments and result as
+ * dmrg_heff_apply, but every intermediate (the uploaded MPO pair, theta in
+ * the contraction precision, T1, T2, y before conversion, penalty-projector
+ * coefficients) lives in one static 64 MiB device array per GPU that the
+ * library maps when the context is created; nothing is allocated or freed
+ * and no profiling events are recorded.  An operator whose workspace would
+ * exceed 64 MiB is rejected with a status and a message naming the size
+ * (a real two-site H_eff with d=2, w=5 fits up to about D=450 in FP64 and
+ * D=610 in FP32).  All
+ * calls on one GPU share that array and it keeps the last call's contents
+ * (nothing in it needs to start at zero): calls that overlap on one GPU,
+ * from several contexts, streams or threads, race. */
+int dmrg_heff_apply_noalloc(dmrg_ctx* ctx, const dmrg_heff_desc* op, const double* x, double* y);
+
 /* <x|H_eff|x> / <x|x> in the operator's precision. */
 int dmrg_rayleigh(dmrg_ctx* ctx, const dmrg_heff_desc* op, const double* x, double* value);
 
