@@ -40,6 +40,7 @@ class Stats:
         self.truncation = []      # (bond, keep, discarded weight)
         self.svd_guided = []      # 1 where the guided subspace split produced the result
         self.svd_resid = []       # its measured weight outside the subspace (-1: not attempted)
+        self.bond_shapes = []     # (bond, Dl, d1, d2, Dr) of every two-site local solve
 
     def reset(self):
         self.__init__()
@@ -113,6 +114,7 @@ def local_ground(eng, L, R, mpo, psi, i, niter, prec, stats=None, tol=LANCZOS_TO
     e, it = eng.lanczos(desc, th, max(niter, LANCZOS_CAP), tol, res_tol)
     if stats is not None:
         stats.lanczos_iters.append(it)
+        stats.bond_shapes.append((i,) + tuple(dims))
     return e, th, dims
 
 
@@ -357,6 +359,74 @@ def random_mps(eng, dims, m, field=1, rng=None):
     position(eng, psi, n)
     normalize(eng, psi)
     return psi
+
+
+# ---------------------------------------------------------------------------
+# MPS sums (mpsalgebra.py:143-232)
+# ---------------------------------------------------------------------------
+def mps_sum_many(eng, states, cutoff=0.0, maxdim=None):
+    """Direct sum of K states (sum_many, mpsalgebra.py:143-223): interior
+    sites block-diagonal in the link spaces, the boundary sites concatenated
+    along their one link; then one truncating left-to-right SVD sweep from
+    site 1 to site N (position(N, cutoff, maxdim)), as the reference does."""
+    states = list(states)
+    if not states:
+        raise ValueError("mps_sum: no states given")
+    n = states[0].length()
+    if any(s.length() != n for s in states):
+        raise ValueError("mps_sum: states of different lengths")
+    if any(s.phys_dims() != states[0].phys_dims() for s in states):
+        raise ValueError("mps_sum: different physical dimensions")
+    field = max(s.field for s in states)
+    states = [s.as_complex() if field and not s.field else s for s in states]
+    npl = 2 if field else 1
+    tensors = []
+    for i in range(1, n + 1):
+        Ts = [s.A(i) for s in states]
+        d = Ts[0].shape[2]
+        Dl = 1 if i == 1 else sum(t.shape[1] for t in Ts)
+        Dr = 1 if i == n else sum(t.shape[3] for t in Ts)
+        out = eng.zeros((npl, Dl, d, Dr))
+        lo = ro = 0
+        for t in Ts:
+            _, a, _, b = t.shape
+            ls = slice(0, 1) if i == 1 else slice(lo, lo + a)
+            rs = slice(0, 1) if i == n else slice(ro, ro + b)
+            out[:, ls, :, rs] = t
+            lo += a
+            ro += b
+        tensors.append(out)
+    res = DeviceMPS(tensors, center=1, field=field)
+    position(eng, res, n, cutoff, maxdim)
+    return res
+
+
+def mps_sum(eng, a, b, cutoff=0.0, maxdim=None):
+    """a + b (sum, mpsalgebra.py:226-232)."""
+    return mps_sum_many(eng, [a, b], cutoff, maxdim)
+
+
+def scaled(psi, factor):
+    """factor * psi as a new state (the scalar goes into the centre site,
+    or site 1)."""
+    f = complex(factor)
+    p = psi.as_complex() if (f.imag != 0 and not psi.field) else psi.copy()
+    scale_site(p, p.center or 1, f)
+    return p
+
+
+def gram_schmidt(eng, wfs, cutoff=0.0, maxdim=None):
+    """Sequential Gram-Schmidt of MPS (Chain._gram_schmidt, chain.py:1233-1241):
+    wf_i <- normalise(wf_i - sum over j < i, one at a time, of wf_j <wf_j|wf_i>),
+    each subtraction an MPS sum truncated to cutoff / maxdim."""
+    wfs = list(wfs)
+    for i in range(1, len(wfs)):
+        for j in range(i):
+            c = overlap(eng, wfs[j], wfs[i])
+            wf = mps_sum(eng, wfs[i], scaled(wfs[j], -c), cutoff, maxdim)
+            normalize(eng, wf)
+            wfs[i] = wf
+    return wfs
 
 
 # ---------------------------------------------------------------------------

@@ -28,7 +28,7 @@ import numpy as np
 from . import algorithms as alg
 from . import sites as _sites
 from .engine import PREC_NAMES, get_engine
-from .mpo import build_mpo
+from .mpo import build_mpo, hamiltonian_mpo
 from .mps import DeviceMPS
 
 BUILD_CUTOFF = 1e-14   # chain.py:38 (MPO build cutoff; the automaton MPO is exact)
@@ -83,7 +83,7 @@ class Chain:
         self.verbose = bool(verbose)
 
     def set_hamiltonian(self, terms):
-        self.H = build_mpo(self.site_types, terms)
+        self.H = hamiltonian_mpo(self.site_types, terms, BUILD_CUTOFF)
         self.H_terms = list(terms)
         self.have_H = True
         self._wf0_energy = None
@@ -136,6 +136,14 @@ class Chain:
 
     def apply_operator(self, terms, wf):
         return self._apply_mpo(build_mpo(self.site_types, terms), wf)
+
+    def sum_mps(self, wf1, wf2):
+        """wf1 + wf2 truncated to the session's cutoff / maxm (chain.py:300-301)."""
+        return alg.mps_sum(self.eng, wf1, wf2, self.cutoff, self.maxm)
+
+    def _gram_schmidt(self, wfs):
+        """chain.py:1233-1241."""
+        return alg.gram_schmidt(self.eng, wfs, self.cutoff, self.maxm)
 
     def tdvp_step(self, H, wf, dt, num_center=2):
         """(H, wf, dt) argument order as chain.py:397-420."""
@@ -270,8 +278,6 @@ class Chain:
         (energies <psi|H|psi>, fluctuations <H^2> - <H>^2, wave functions)."""
         if not self.have_H:
             raise RuntimeError("Chain.excited_states called before set_hamiltonian")
-        if do_gram_schmidt:
-            raise NotImplementedError("do_gram_schmidt needs MPS sums, which this backend does not provide")
         if self.wf0 is None:
             self.gs_energy()
         psi0 = self.wf0.copy()
@@ -292,22 +298,39 @@ class Chain:
                                        prec, self.stats, res_tol=RES_TOL.get(prec, 0.0))
             alg.normalize(self.eng, psi1)
             wfs.append(psi1)
+        if do_gram_schmidt:
+            wfs = self._gram_schmidt(wfs)
         energies, fluct = [], []
         for wf in wfs:
             fluct.append(self._energy_fluctuation(wf, self.H))
             energies.append((alg.expectation(self.eng, wf, self.H, wf) / alg.overlap(self.eng, wf, wf)).real)
         return energies, fluct, wfs
 
+    def last_sweep_truncation(self):
+        """[(bond, keep, discarded weight)] of every split of the last
+        sweep: the discarded weight is the reference's Spectrum.truncerr
+        (src/dmrgpy/pyitensor/svd.py:16-29,195), which its dmrg computes and
+        drops."""
+        return list(self._last_truncation)
+
+    def truncation_weight(self):
+        """Total discarded weight of the last sweep (sum over its splits)."""
+        return float(sum(t[2] for t in self._last_truncation))
+
+    _last_truncation = ()
+
     def _dmrg(self, psi, H, callback=None):
         energy = None
         self.sweep_energies = []
         for k in range(self.nsweeps):
+            t0 = len(self.stats.truncation)
             maxdim = self.maxdim_schedule[min(k, len(self.maxdim_schedule) - 1)] if self.maxdim_schedule \
                 else self.maxm
             prec = self._prec(k)
             energy = alg.dmrg_sweep(self.eng, psi, H, maxdim, self.cutoff, self.niter, prec, self.stats,
                                     res_tol=RES_TOL.get(prec, 0.0))
             self.sweep_energies.append(energy)
+            self._last_truncation = tuple(self.stats.truncation[t0:])
             if self.verbose:
                 print("sweep {}: energy = {}".format(k, energy))
             if callback is not None:

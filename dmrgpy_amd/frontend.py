@@ -6,7 +6,10 @@ with ``itensor_version=`` and then calls ``set_hamiltonian``,
 ``gs_energy``, ``get_gs`` and ``vev`` on MultiOperators.  This module keeps
 those names and call semantics for the backend value ``"mi355x"``; the
 session is ``dmrgpy_amd.chain.Chain``.  Only the hot-path calls are
-provided (ground state, expectation values, TDVP time evolution).
+provided (ground state, expectation values, TDVP time evolution).  Chains:
+``Spin_Chain``, ``Fermionic_Chain``, ``Spinful_Fermionic_Chain`` (2n
+interleaved spinless sites) and ``Spinful_Fermionic_Chain_Native`` (n d=4
+sites).
 """
 
 import numpy as np
@@ -162,3 +165,24 @@ class Spinful_Fermionic_Chain_Native(Many_Body_Chain):
         self.Ndn = [g("Ndn", i) for i in range(n)]
         self.Ntot = [self.Nup[i] + self.Ndn[i] for i in range(n)]
         self.Sz = [0.5 * self.Nup[i] - 0.5 * self.Ndn[i] for i in range(n)]
+
+
+class Spinful_Fermionic_Chain(Fermionic_Chain):
+    """fermionchain.py:210-233: n orbitals as 2n interleaved spinless-fermion
+    sites (up on site 2i, down on site 2i+1, 0-based), with the spin,
+    pairing and number operators built from them."""
+
+    def __init__(self, n, **kwargs):
+        Fermionic_Chain.__init__(self, 2 * n, **kwargs)
+        C, Cdag, N = self.C, self.Cdag, self.N
+        self.Sx = [0.5 * Cdag[2 * i] * C[2 * i + 1] + 0.5 * Cdag[2 * i + 1] * C[2 * i] for i in range(n)]
+        self.Sy = [-0.5j * Cdag[2 * i] * C[2 * i + 1] + 0.5j * Cdag[2 * i + 1] * C[2 * i] for i in range(n)]
+        self.Sz = [0.5 * N[2 * i] + (-0.5) * N[2 * i + 1] for i in range(n)]
+        self.Delta = [0.5 * C[2 * i] * C[2 * i + 1] for i in range(n)]
+        self.Cup = [C[2 * i] for i in range(n)]
+        self.Cdagup = [Cdag[2 * i] for i in range(n)]
+        self.Cdn = [C[2 * i + 1] for i in range(n)]
+        self.Cdagdn = [Cdag[2 * i + 1] for i in range(n)]
+        self.Nup = [N[2 * i] for i in range(n)]
+        self.Ndn = [N[2 * i + 1] for i in range(n)]
+        self.Ntot = [self.Nup[i] + self.Ndn[i] for i in range(n)]

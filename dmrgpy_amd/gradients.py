@@ -50,6 +50,8 @@ def term_gradients(eng, psi, mpo, nterms, prec=0):
 def contract_term_gradients(G, mpo, nterms):
     """dE/dc_t = source_t * sum(G_s[wl_t, wr_t] * M_t) over the MPO elements
     that carry term t's coefficient."""
+    if not mpo.coupling_sites and nterms > 0:
+        raise ValueError("this MPO has no per-term bookkeeping (SVD-compressed): build it with mpo.build_mpo")
     g = np.zeros(nterms, dtype=complex)
     for ti, s, fr, to, m, source in mpo.coupling_sites:
         Gs = G[s]
@@ -153,7 +155,7 @@ def simpson_weights(nt, dt):
 
 
 def tdvp_correlator_gradient(eng, codes, terms, psi1, psi2, e0, dt, nt, maxdim, cutoff=1e-12, niter=50,
-                             checkpoint=10, prec=0):
+                             checkpoint=10, prec=0, value_only=False):
     """C(T) = <psi2| psi(T)> and dC(T)/dc_t for every input term t of H.
 
     psi(t) = exp(-i (H - e0) t) psi1 is evolved by the two-site TDVP steps of
@@ -168,7 +170,8 @@ def tdvp_correlator_gradient(eng, codes, terms, psi1, psi2, e0, dt, nt, maxdim, 
     checkpoints stored every ``checkpoint`` steps of the forward pass, so at
     most nt / checkpoint + checkpoint + 2 states are held at once.  The
     integrand is the transition gradient <phi|dH/dc_t|psi> of the MPO
-    gradient kernel.  Returns (C(T), complex array over the terms)."""
+    gradient kernel.  Returns (C(T), complex array over the terms), or
+    (C(T), None) with value_only (the forward evolution alone)."""
     from .mpo import build_mpo
     nterm = len(terms)
     Hs = build_mpo(codes, list(terms) + [(-e0, [("Id", 1)])], force_complex=False)
@@ -195,9 +198,11 @@ def tdvp_correlator_gradient(eng, codes, terms, psi1, psi2, e0, dt, nt, maxdim, 
     psi = p1
     for k in range(1, nt + 1):
         step(psi, dt, n1)
-        if k % checkpoint == 0 or k == nt:
+        if not value_only and (k % checkpoint == 0 or k == nt):
             ckpt[k] = psi.copy()
     C = alg.overlap(eng, p2, psi)
+    if value_only:
+        return C, None
     g = np.zeros(nterm + 1, dtype=complex)
     phi = p2
     marks = sorted(ckpt)
@@ -224,6 +229,138 @@ def xxz_correlator_gradients(eng, psi1, psi2, e0, J, Delta, dt, nt, maxdim, cuto
                                     maxdim, cutoff, checkpoint=checkpoint, prec=prec)
     gx, gy, gz = g[0::3], g[1::3], g[2::3]
     return C, gx + gy + Delta * gz, J * gz
+
+
+# ---------------------------------------------------------------------------
+# E0 as a differentiable function of the couplings of any term list
+# ---------------------------------------------------------------------------
+def coupled_terms(coeffs, operators):
+    """H = sum_k c_k O_k as a session term list: each O_k is a term list
+    (its own coefficients kept as written) scaled by c_k."""
+    t = []
+    for c, op in zip(coeffs, operators):
+        for a, f in op:
+            t.append((complex(c) * complex(a), list(f)))
+    return t
+
+
+def operator_expectations(eng, psi, codes, operators, prec=0):
+    """<O_k> of the normalised psi for every term list O_k, from one
+    backward pass of the MPO-gradient kernels (the gradient of
+    <psi|sum_k c_k O_k|psi> at c = 1)."""
+    from .mpo import build_mpo
+    ones = np.ones(len(operators))
+    terms = coupled_terms(ones, operators)
+    g = term_gradients(eng, psi, build_mpo(codes, terms), len(terms), prec)
+    out = np.zeros(len(operators), dtype=complex)
+    t = 0
+    for k, op in enumerate(operators):
+        for a, _ in op:
+            out[k] += complex(a) * g[t]
+            t += 1
+    return out
+
+
+def _autograd_terms():
+    import torch
+
+    class CoupledGroundStateEnergy(torch.autograd.Function):
+        """E0(c) of H = sum_k c_k O_k by DMRG on the device (warm-started
+        from the session's current state); backward: dE0/dc_k = <O_k>
+        (Hellmann-Feynman) by the MPO-gradient kernels."""
+
+        @staticmethod
+        def forward(ctx, c, operators, chain):
+            cn = c.detach().cpu().double().numpy()
+            chain.set_hamiltonian(coupled_terms(cn, operators))
+            e = chain.gs_energy()
+            g = operator_expectations(chain.eng, chain.gs_wavefunction(), chain.site_types, operators).real
+            ctx.save_for_backward(torch.from_numpy(g))
+            return torch.tensor(e, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, grad):
+            (g,) = ctx.saved_tensors
+            return grad * g.to(grad.device), None, None
+
+    return CoupledGroundStateEnergy
+
+
+def coupled_ground_state_energy(c, operators, chain):
+    """Differentiable E0(c) for H = sum_k c_k O_k with O_k arbitrary term
+    lists (torch tensor c in, scalar tensor out); the session keeps the
+    ground state, so successive calls warm-start DMRG."""
+    return _autograd_terms().apply(c, operators, chain)
+
+
+def xxz_bond_operators(n):
+    """[O_xy_1, .., O_xy_{n-1}, O_zz_1, .., O_zz_{n-1}] with
+    O_xy_i = Sx_i Sx_{i+1} + Sy_i Sy_{i+1}, O_zz_i = Sz_i Sz_{i+1}: the XXZ
+    chain is H = sum_i a_i O_xy_i + b_i O_zz_i with a_i = J_i,
+    b_i = J_i Delta_i."""
+    xy = [[(1.0, [("Sx", i + 1), ("Sx", i + 2)]), (1.0, [("Sy", i + 1), ("Sy", i + 2)])] for i in range(n - 1)]
+    zz = [[(1.0, [("Sz", i + 1), ("Sz", i + 2)])] for i in range(n - 1)]
+    return xy + zz
+
+
+def xxz_bond_correlations(eng, psi, n, prec=0):
+    """(<XX+YY>_i, <ZZ>_i) for the n-1 bonds of the normalised psi."""
+    v = operator_expectations(eng, psi, [2] * n, xxz_bond_operators(n), prec).real
+    return v[:n - 1], v[n - 1:]
+
+
+def fit_couplings(chain, operators, targets, c0, scale_index=None, scale=None, steps=200, tol=1e-9,
+                  callback=None):
+    """Couplings c of H(c) = sum_k c_k O_k whose ground state has <O_k> = t_k.
+
+    E0(c) is concave (a minimum of functions linear in c) with gradient
+    <O_k>_c (Hellmann-Feynman), so F(c) = E0(c) - c.t is concave and its
+    maximiser reproduces the targets.  E0 is homogeneous of degree one, so
+    the overall scale is not fixed by correlations: the penalty
+    (mean(c[scale_index]) - scale)^2 pins it (defaults: all couplings, the
+    mean of c0).  L-BFGS on -F; every evaluation re-solves DMRG warm-started
+    from the previous ground state with the chain's sweep settings, and the
+    gradient comes from one backward pass of the device MPO-gradient
+    kernels.  Returns (c, max |<O> - t|, history of that maximum)."""
+    from scipy.optimize import minimize
+    targets = np.asarray(targets, float)
+    c0 = np.asarray(c0, float)
+    idx = np.arange(len(c0)) if scale_index is None else np.asarray(scale_index)
+    m0 = c0[idx].mean() if scale is None else float(scale)
+    hist = []
+
+    def fun(c):
+        chain.set_hamiltonian(coupled_terms(c, operators))
+        e = chain.gs_energy()
+        ex = operator_expectations(chain.eng, chain.gs_wavefunction(), chain.site_types, operators).real
+        res = ex - targets
+        hist.append(float(np.abs(res).max()))
+        if callback is not None:
+            callback(len(hist), c, hist[-1])
+        pen = c[idx].mean() - m0
+        grad = -res
+        grad[idx] += 2.0 * pen / len(idx)
+        return -(e - c @ targets) + pen ** 2, grad
+
+    r = minimize(fun, c0, jac=True, method="L-BFGS-B",
+                 options={"maxiter": steps, "maxfun": steps, "gtol": tol, "ftol": 1e-16, "maxcor": 30})
+    fun(r.x)
+    return r.x, hist[-1], hist
+
+
+def fit_xxz_couplings(chain, t_xy, t_zz, J0, Delta0, scale=None, steps=200, tol=1e-9, callback=None):
+    """(J, Delta, max residual, history): the XXZ couplings whose ground
+    state has bond correlations <XX+YY>_i = t_xy_i and <ZZ>_i = t_zz_i, by
+    fit_couplings over a_i = J_i, b_i = J_i Delta_i with the scale of the
+    J_i pinned to `scale` (default mean(J0))."""
+    n = len(J0) + 1
+    J0 = np.asarray(J0, float)
+    c0 = np.concatenate([J0, J0 * np.asarray(Delta0, float)])
+    c, resid, hist = fit_couplings(chain, xxz_bond_operators(n), np.concatenate([t_xy, t_zz]), c0,
+                                   scale_index=np.arange(n - 1), scale=J0.mean() if scale is None else scale,
+                                   steps=steps, tol=tol, callback=callback)
+    a, b = c[:n - 1], c[n - 1:]
+    return a, b / a, resid, hist
 
 
 # ---------------------------------------------------------------------------

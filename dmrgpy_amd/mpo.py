@@ -231,3 +231,69 @@ def build_mpo(codes, terms, force_complex=False):
         planes = [Wc.real] + ([Wc.imag] if cplx else [])
         W.append(np.ascontiguousarray(np.stack(planes), dtype=np.float64))
     return MPO(W, codes, coupling_sites)
+
+
+# ---------------------------------------------------------------------------
+# SVD compression (to_mpo, src/dmrgpy/pyitensor/mpobuilder.py:92-103)
+# ---------------------------------------------------------------------------
+def _keep_count(s, cutoff, maxdim=None):
+    """The reference's keep rule (svd.py:32-68) with mindim 1: drop the
+    largest tail of s^2 whose normalised weight stays <= cutoff."""
+    p = np.asarray(s, float) ** 2
+    total = p.sum()
+    n = len(p)
+    if total <= 0:
+        return 1
+    tail = np.concatenate(([0.0], np.cumsum(p[::-1] / total)))[:n]   # weight of the j smallest, j < n
+    j = int(np.nonzero(tail <= cutoff)[0].max())
+    keep = n - j
+    if maxdim:
+        keep = min(keep, int(maxdim))
+    return max(1, keep)
+
+
+def compress_mpo(mpo, cutoff=1e-14, maxdim=None):
+    """Bidirectional truncating SVD sweeps over the MPO viewed as an MPS with
+    site index (s_out, s_in): left to right then right to left, each bond
+    kept by the reference's rule at `cutoff` (to_mpo compresses its sum of
+    terms exactly this way, mpobuilder.py:92-103 with the 1e-14 build
+    cutoff of chain.py:38).  Reaches the minimal bond dimension for
+    structures the automaton does not share, e.g. exponentially decaying
+    long-range couplings (automaton w = O(N), compressed w = 5).  The result
+    has no per-term bookkeeping (gradients use the automaton MPO)."""
+    Ws = [w[0] + 1j * w[1] if w.shape[0] == 2 else w[0].astype(complex) for w in mpo.W]
+    cplx = mpo.complex
+    n = len(Ws)
+    # T[s]: [wl, d_out, d_in, wr]
+    T = [w.transpose(0, 2, 3, 1) for w in Ws]
+    if n > 1:
+        for s in range(n - 1):
+            wl, do, di, wr = T[s].shape
+            U, S, Vh = np.linalg.svd(T[s].reshape(wl * do * di, wr), full_matrices=False)
+            k = _keep_count(S, cutoff, maxdim)
+            T[s] = U[:, :k].reshape(wl, do, di, k)
+            T[s + 1] = np.tensordot(S[:k, None] * Vh[:k], T[s + 1], axes=([1], [0]))
+        for s in range(n - 1, 0, -1):
+            wl, do, di, wr = T[s].shape
+            U, S, Vh = np.linalg.svd(T[s].reshape(wl, do * di * wr), full_matrices=False)
+            k = _keep_count(S, cutoff, maxdim)
+            T[s] = Vh[:k].reshape(k, do, di, wr)
+            T[s - 1] = np.tensordot(T[s - 1], U[:, :k] * S[:k], axes=([3], [0]))
+    W = []
+    for t in T:
+        Wc = t.transpose(0, 3, 1, 2)
+        planes = [Wc.real] + ([Wc.imag] if cplx else [])
+        W.append(np.ascontiguousarray(np.stack(planes), dtype=np.float64))
+    return MPO(W, mpo.codes, None)
+
+
+def hamiltonian_mpo(codes, terms, cutoff=1e-14):
+    """The MPO the session sweeps with: the automaton MPO, or its SVD
+    compression when that has a smaller largest bond dimension (the
+    automaton is kept otherwise: it is exact, minimal for sums of local and
+    two-site terms, and sparse, which the H_eff kernels exploit)."""
+    H = build_mpo(codes, terms)
+    if len(codes) < 3 or max(H.bond_dims()) <= 8:
+        return H
+    Hc = compress_mpo(H, cutoff)
+    return Hc if max(Hc.bond_dims()) < max(H.bond_dims()) else H

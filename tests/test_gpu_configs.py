@@ -1,0 +1,311 @@
+"""The benchmark configurations at their full sizes (GPU).
+
+Each test runs one configuration of the benchmark plan end to end through
+the session and pins it with what the domain fixes at that size: the FP64
+path against the reduced-precision path from the same seeded state (the
+north-star bound |dE|/N < 1e-9), isometry of every tensor a sweep leaves
+behind, closed forms (free fermions), Hellmann-Feynman values computed with
+the reference's own ``vev`` semantics (src/dmrgpy/pyitensor/chain.py:277-287)
+and central finite differences.
+
+* C2: open S=1/2 Heisenberg N=100, D ramped 64 -> 512 over 8 sweeps, FP32
+  contractions + FP64 Lanczos/SVD + Rayleigh vs FP64;
+* C3: spinful Hubbard N=64 (native d=4 sites), t=1, U=4, ramp to D=1024:
+  energy, double occupancy and truncation weight of an FP32 sweep vs an
+  FP64 sweep from the same state;
+* C4: random-bond XXZ N=64, D=256: all 126 dE0/dJ_i, dE0/dDelta_i vs
+  Hellmann-Feynman vev and finite differences; couplings of a hidden XXZ
+  chain recovered from its bond correlations;
+* C5: S=1 Heisenberg N=128, D cap 1024: TDVP quench C(t) and the
+  checkpointed adjoint gradient of C(T) vs finite differences.
+
+Every test prints the measured deviations.
+"""
+
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _session(codes, terms, maxm, precision="fp64"):
+    from dmrgpy_amd.chain import Chain
+    ch = Chain(codes, precision=precision, field="real")
+    ch.set_sweep_params(maxm, 1, 1e-12, 0.0)
+    ch.set_verbose(False)
+    ch.set_hamiltonian(terms)
+    return ch
+
+
+def _host(t):
+    a = t.double().cpu().numpy()
+    return a[0] + 1j * a[1] if a.shape[0] == 2 else a[0]
+
+
+def _assert_right_canonical(psi, tol):
+    """Sites 2..N of a state whose centre is site 1: B B^dag = I."""
+    assert psi.center == 1
+    worst = 0.0
+    for i in range(2, psi.length() + 1):
+        B = _host(psi.A(i))
+        Bm = B.reshape(B.shape[0], -1)
+        worst = max(worst, np.abs(Bm @ Bm.conj().T - np.eye(Bm.shape[0])).max())
+    print("   max |B B^dag - I| over the right-canonical sites:", worst, flush=True)
+    assert worst < tol
+
+
+# ---------------------------------------------------------------------------
+# C2
+# ---------------------------------------------------------------------------
+def test_c2_heisenberg_n100_ramp_fp32_vs_fp64():
+    from dmrgpy_amd.multioperator import heisenberg_terms
+    n = 100
+    sched = [64, 128, 192, 256, 320, 384, 448, 512]
+    terms = heisenberg_terms(n)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        ch = _session([2] * n, terms, 512, precision=prec)
+        ch.set_sweep_params(512, len(sched), 1e-12, 0.0)
+        ch.maxdim_schedule = sched
+        np.random.seed(0)
+        ch.wf0 = ch._default_mps()
+        e = ch.gs_energy()            # FP32 runs end with the FP64 Rayleigh quotient
+        psi = ch.gs_wavefunction()
+        print("C2", prec, "E", repr(e), "max bond", max(psi.bond_dims()), "sweeps", ch.sweep_energies,
+              "t {:.1f}s".format(time.perf_counter() - t0), flush=True)
+        out[prec] = (e, psi, ch)
+    err = abs(out["fp32"][0] - out["fp64"][0]) / n
+    print("C2 |E_fp32 - E_fp64|/N =", err, flush=True)
+    assert err < 1e-9
+    for prec in ("fp64", "fp32"):
+        _assert_right_canonical(out[prec][1], 1e-12)
+    # the FP64 path's last local eigenvalue against <psi|H|psi> of the state
+    # it leaves (they differ by the effect of the last truncation only)
+    from dmrgpy_amd import algorithms as alg
+    e64, psi64, ch64 = out["fp64"]
+    ray = (alg.expectation(ch64.eng, psi64, ch64.H, psi64) / alg.overlap(ch64.eng, psi64, psi64)).real
+    print("C2 fp64 last local eigenvalue vs <H>: |dE|/N", abs(ray - e64) / n, flush=True)
+    assert abs(ray - e64) / n < 1e-11
+
+
+def test_c2_xx_chain_n100_free_fermions():
+    """The XX chain (Delta = 0) of the same length against its closed form
+    (Jordan-Wigner free fermions): FP32 contractions with the FP64 Rayleigh
+    quotient, D ramped to 512."""
+    from dmrgpy_amd.multioperator import heisenberg_terms
+    from oracle import ed
+    n = 100
+    terms = heisenberg_terms(n, 1.0, 0.0)
+    e_ff, _ = ed.free_fermion_xx(n)
+    ch = _session([2] * n, terms, 512, precision="fp32")
+    sched = [64, 128, 256, 512, 512, 512]
+    ch.set_sweep_params(512, len(sched), 1e-12, 0.0)
+    ch.maxdim_schedule = sched
+    np.random.seed(2)
+    ch.wf0 = ch._default_mps()
+    e = ch.gs_energy()
+    err = (e - e_ff) / n
+    print("C2 XX chain: E", e, "free fermions", e_ff, "(E - E_ff)/N", err, flush=True)
+    assert -1e-12 < err < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# C3
+# ---------------------------------------------------------------------------
+def test_c3_hubbard_n64_d1024_fp32_vs_fp64():
+    from dmrgpy_amd import algorithms as alg
+    from dmrgpy_amd.chain import RES_TOL
+    from dmrgpy_amd.mpo import build_mpo
+    from dmrgpy_amd.multioperator import hubbard_terms
+    n, D = 64, 1024
+    ch = _session([1] * n, hubbard_terms(n, 1.0, 4.0), D)
+    eng = ch.eng
+    np.random.seed(0)
+    psi = ch._default_mps(maxm=128)
+    t0 = time.perf_counter()
+    for m in (128, 256, 512, 1024):
+        e = alg.dmrg_sweep(eng, psi, ch.H, m, 1e-12, 2, 1, None, res_tol=RES_TOL[1])
+        print("C3 ramp D={} E={:.10f} max bond {} t={:.1f}s".format(m, e, max(psi.bond_dims()),
+                                                                  time.perf_counter() - t0), flush=True)
+    docc = build_mpo(ch.site_types, [(1.0 / n, [("Nup", i), ("Ndn", i)]) for i in range(1, n + 1)])
+    res = {}
+    for prec in (1, 0):
+        p = psi.copy()
+        stats = alg.Stats()
+        alg.dmrg_sweep(eng, p, ch.H, D, 1e-12, 2, prec, stats, res_tol=RES_TOL.get(prec, 0.0))
+        nrm = alg.overlap(eng, p, p).real
+        e = (alg.expectation(eng, p, ch.H, p) / nrm).real
+        d = (alg.expectation(eng, p, docc, p) / nrm).real
+        tw = sum(t[2] for t in stats.truncation)
+        # every split is cutoff-limited or maxdim-limited (keep == D)
+        assert all(t[2] <= 1e-12 or t[1] == D for t in stats.truncation)
+        res[prec] = (e, d, tw, max(p.bond_dims()), p)
+        print("C3 {} sweep: E {:.12f} <n_up n_dn> {:.12f} truncation weight {:.3e} max bond {} t={:.1f}s".format(
+            "fp32" if prec else "fp64", e, d, tw, res[prec][3], time.perf_counter() - t0), flush=True)
+    (e32, d32, t32, b32, p32), (e64, d64, t64, b64, p64) = res[1], res[0]
+    print("C3 |dE|/N", abs(e32 - e64) / n, "|d docc|", abs(d32 - d64), "truncation weight ratio", t32 / t64,
+          flush=True)
+    assert abs(e32 - e64) / n < 1e-9
+    assert abs(d32 - d64) < 1e-6
+    assert 0.5 < t32 / t64 < 2.0
+    # the FP32 sweep reaches the benchmarked shapes (Gram orders in the thousands)
+    assert b32 >= 600 and b64 >= 600
+    _assert_right_canonical(p32, 1e-10)
+    # particle-hole symmetric Hubbard chain at half filling: <n> = 1 per site
+    dens = build_mpo(ch.site_types, [(1.0 / n, [(nm, i)]) for i in range(1, n + 1) for nm in ("Nup", "Ndn")])
+    nbar = (alg.expectation(eng, p64, dens, p64) / alg.overlap(eng, p64, p64)).real
+    print("C3 <n> per site", nbar, flush=True)
+    assert abs(nbar - 1.0) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# C4
+# ---------------------------------------------------------------------------
+C4_N, C4_D = 64, 256
+
+
+def _c4_couplings(seed=4):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.5, 1.5, C4_N - 1), rng.uniform(0.5, 1.5, C4_N - 1)
+
+
+def _c4_solve(J, Dz, wf=None, nsw=8):
+    from dmrgpy_amd.chain import Chain
+    from dmrgpy_amd.gradients import xxz_terms
+    ch = Chain([2] * C4_N, precision="fp32", field="real")
+    ch.set_sweep_params(C4_D, nsw, 1e-12, 0.0)
+    ch.precision_schedule = ["fp32"] * (nsw - 1) + ["fp64"]
+    ch.set_hamiltonian(xxz_terms(J, Dz))
+    if wf is not None:
+        ch.set_wavefunction(wf.copy())
+    else:
+        np.random.seed(1)
+    return ch, ch.gs_energy()
+
+
+def test_c4_xxz_gradients_vs_hellmann_feynman_and_fd():
+    from dmrgpy_amd.gradients import xxz_gradients
+    n = C4_N
+    J, Dz = _c4_couplings()
+    t0 = time.perf_counter()
+    ch, e = _c4_solve(J, Dz)
+    psi = ch.gs_wavefunction()
+    gJ, gD = xxz_gradients(ch.eng, psi, J, Dz)
+    print("C4 E0", e, "max bond", max(psi.bond_dims()), "t {:.1f}s".format(time.perf_counter() - t0), flush=True)
+    # Hellmann-Feynman through the session's vev (chain.py:277-287):
+    # dE/dJ_i = <XX+YY>_i + Delta_i <ZZ>_i, dE/dDelta_i = J_i <ZZ>_i
+    hf_J, hf_D = np.zeros(n - 1), np.zeros(n - 1)
+    for i in range(n - 1):
+        xy = ch.vev([(1.0, [("Sx", i + 1), ("Sx", i + 2)]), (1.0, [("Sy", i + 1), ("Sy", i + 2)])], psi).real
+        zz = ch.vev([(1.0, [("Sz", i + 1), ("Sz", i + 2)])], psi).real
+        hf_J[i] = xy + Dz[i] * zz
+        hf_D[i] = J[i] * zz
+    dev = max(np.abs(gJ - hf_J).max(), np.abs(gD - hf_D).max())
+    print("C4 max |grad - Hellmann-Feynman vev| over 126 couplings:", dev, flush=True)
+    assert dev < 1e-10
+    # E0 is the sum of the coupling-weighted bond energies (Euler: E0 is
+    # homogeneous of degree one in the couplings J_i and J_i Delta_i)
+    assert abs(J @ hf_J - e) < 1e-9 * n
+    # central differences, fourth-order stencil, warm-started DMRG (FP64 final sweep)
+    h = 1e-3
+    for kind, i in (("J", 0), ("J", n // 2), ("D", 5), ("D", n - 3)):
+        vals = []
+        for k in (2, 1, -1, -2):
+            Jx, Dx = J.copy(), Dz.copy()
+            (Jx if kind == "J" else Dx)[i] += k * h
+            vals.append(_c4_solve(Jx, Dx, psi, 4)[1])
+        fd = (8 * (vals[1] - vals[2]) - (vals[0] - vals[3])) / (12 * h)
+        g = gJ[i] if kind == "J" else gD[i]
+        rel = abs(fd - g) / abs(g)
+        print("C4 d E0 / d{}_{}: kernel {:.12f} finite differences {:.12f} rel {:.2e}".format(kind, i + 1, g, fd, rel),
+              flush=True)
+        assert rel < 1e-6
+
+
+def test_c4_fit_hidden_xxz_couplings():
+    """Couplings of a hidden seeded XXZ chain recovered from its bond
+    correlations <XX+YY>_i and <ZZ>_i (126 targets for 126 couplings) by
+    maximising the concave Legendre function E0(a, b) - a.t_xy - b.t_zz
+    (a_i = J_i, b_i = J_i Delta_i) with the device gradient kernels."""
+    from dmrgpy_amd.gradients import fit_xxz_couplings, xxz_bond_correlations
+    n = C4_N
+    Jh, Dh = _c4_couplings(seed=44)
+    ch, _ = _c4_solve(Jh, Dh)
+    t_xy, t_zz = xxz_bond_correlations(ch.eng, ch.gs_wavefunction(), n)
+    fit = _c4_solve(np.full(n - 1, Jh.mean()), np.ones(n - 1))[0]
+    fit.set_sweep_params(C4_D, 2, 1e-12, 0.0)      # each evaluation: warm-started FP32 + FP64 sweep
+    fit.precision_schedule = ["fp32", "fp64"]
+    t0 = time.perf_counter()
+    J, Dz, resid, hist = fit_xxz_couplings(fit, t_xy, t_zz, np.full(n - 1, Jh.mean()), np.ones(n - 1),
+                                           scale=Jh.mean(), steps=200, tol=1e-9)
+    print("C4 fit: {} evaluations, {:.1f}s, max |corr - target| {:.2e}, max |J - J*| {:.2e}, "
+          "max |Delta - Delta*| {:.2e}".format(len(hist), time.perf_counter() - t0, resid, np.abs(J - Jh).max(),
+                                               np.abs(Dz - Dh).max()), flush=True)
+    # measured: 202 evaluations reach 2.1e-5 in the correlations; the
+    # couplings of the weakly coupled directions are less well determined
+    # (max |J - J*| 0.11, |Delta - Delta*| 0.019): the Legendre function is
+    # flat along them (small susceptibility)
+    assert resid < 5e-5
+    assert hist[-1] < 1e-3 * hist[0]
+    assert np.abs(J - Jh).max() < 0.2 and np.abs(Dz - Dh).max() < 0.05
+
+
+# ---------------------------------------------------------------------------
+# C5
+# ---------------------------------------------------------------------------
+def test_c5_s1_tdvp_quench_and_adjoint_gradient():
+    from dmrgpy_amd import algorithms as alg
+    from dmrgpy_amd.chain import RES_TOL
+    from dmrgpy_amd.gradients import tdvp_correlator_gradient
+    from dmrgpy_amd.mpo import build_mpo
+    from dmrgpy_amd.multioperator import heisenberg_terms
+    n, D, dt, nt = 128, 1024, 0.05, 10
+    terms = heisenberg_terms(n)
+    ch = _session([3] * n, terms, D)
+    eng = ch.eng
+    np.random.seed(0)
+    psi = ch._default_mps(maxm=64)
+    t0 = time.perf_counter()
+    for m in (64, 128, 256, D):
+        alg.dmrg_sweep(eng, psi, ch.H, m, 1e-12, 2, 1, None, res_tol=RES_TOL[1])
+    alg.dmrg_sweep(eng, psi, ch.H, D, 1e-12, 2, 0, None)
+    e0 = (alg.expectation(eng, psi, ch.H, psi) / alg.overlap(eng, psi, psi)).real
+    print("C5 ground state E0 {:.10f} max bond {} t={:.1f}s".format(e0, max(psi.bond_dims()),
+                                                                   time.perf_counter() - t0), flush=True)
+    site = n // 2
+    A = build_mpo(ch.site_types, [(1.0, [("Sz", site)])])
+    psi1 = alg.apply_mpo(eng, A, psi, 1e-12, D)
+    # C(0) = <Sz Sz> of the ground state (src/dmrgpy/pyitensor/chain.py:520-542)
+    szsz = ch.vev([(1.0, [("Sz", site), ("Sz", site)])], psi).real
+    # quench_tdvp semantics: C(t) = <psi2|psi1(t)>, measured before each step
+    ch.wf0 = psi
+    ch.set_sweep_params(D, 1, 1e-12, 0.0)
+    corr, _ = ch.quench_tdvp(terms, [(1.0, [("Sz", site)])], [(1.0, [("Sz", site)])], nt, dt)
+    print("C5 C(0) {} <SzSz> {} |C(t)| {}".format(corr[0], szsz, np.abs(corr)), "t={:.1f}s".format(
+        time.perf_counter() - t0), flush=True)
+    assert abs(corr[0] - szsz) < 1e-10
+    assert abs(corr[0].imag) < 1e-12
+    assert np.all(np.abs(corr) <= szsz + 1e-10)          # |<psi2|U psi1>| <= |psi1||psi2|
+    # adjoint gradient of C(T) with checkpoints every 5 steps vs central
+    # differences on two couplings J_i of H = sum_i J_i S_i.S_{i+1}
+    psi2 = psi1.copy()
+    C, g = tdvp_correlator_gradient(eng, ch.site_types, terms, psi1, psi2, e0, dt, nt, D, checkpoint=5)
+    gJ = g[0::3] + g[1::3] + g[2::3]
+    print("C5 C(T)", C, "t={:.1f}s".format(time.perf_counter() - t0), flush=True)
+    h = 1e-4
+    for i in (site - 1, site + 3):
+        vals = []
+        for s in (1, -1):
+            tt = [(c * (1 + s * h) if f[0][1] == i + 1 and len(f) == 2 and f[1][1] == i + 2 else c, f)
+                  for c, f in terms]
+            Cs, _ = tdvp_correlator_gradient(eng, ch.site_types, tt, psi1, psi2, e0, dt, nt, D, checkpoint=nt,
+                                             value_only=True)
+            vals.append(Cs)
+        fd = (vals[0] - vals[1]) / (2 * h)
+        rel = abs(fd - gJ[i]) / abs(gJ[i])
+        print("C5 dC(T)/dJ_{}: adjoint {} finite differences {} rel {:.2e}".format(i + 1, gJ[i], fd, rel),
+              flush=True)
+        assert rel < 1e-6

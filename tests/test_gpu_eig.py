@@ -319,3 +319,84 @@ def test_reduction_is_deterministic(engine, field, n):
                 bad += 1
         torch.cuda.synchronize()
         assert bad == 0, f"mode {mode}: {bad} of {reps} identical calls differed"
+
+
+# ---- the Gram orders of the D=1024 Hubbard sweeps (min(Dl d, d Dr) up to
+# 4096): eigenvalues against the constructed spectrum, the eigenvectors the
+# truncation keeps (top 1024) by residual and orthonormality, and the default
+# back-transformation (mode 1) against rocSOLVER's dormtr (mode 2)
+
+def _dmrg_like_spectrum(rng, n, keep=900):
+    """Eigenvalues decaying to 1e-15 of the largest at index `keep` (the
+    weight cut of the 1e-12 truncation), then a noise floor of 1e-18..2e-18."""
+    k = np.arange(n)
+    return np.maximum(10.0 ** (-15.0 * k / keep), 1e-18 * (1.0 + rng.random(n)))
+
+
+@pytest.mark.parametrize("n", [3500, 4096])
+def test_large_gram_orders(engine, n):
+    import time
+    import torch
+    rng = np.random.default_rng(n)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    lam = _dmrg_like_spectrum(rng, n)
+    A = (Q * lam) @ Q.T
+    A = (A + A.T) / 2
+    ref = np.sort(lam)
+    out = {}
+    for mode in (1, 2):
+        t0 = time.perf_counter()
+        w, Z = engine.sym_eig(_dev(engine, A), mode)
+        torch.cuda.synchronize()
+        out[mode] = (w.cpu().numpy(), Z.cpu().numpy())
+        print("order", n, "mode", mode, "sym_eig {:.1f} ms".format(1e3 * (time.perf_counter() - t0)), flush=True)
+    w, Z = out[1]
+    assert np.all(np.diff(w) >= -1e-12)
+    # |A| = 1 and A carries rounding of ~n eps from its construction
+    err_w = np.abs(w - ref).max()
+    top = Z[-1024:]
+    R = A @ top.T - top.T * w[-1024:]
+    err_r = np.abs(R).max()
+    err_o = np.abs(Z @ Z.T - np.eye(n)).max()
+    err_bt = np.abs(out[1][1][-1024:] - out[2][1][-1024:]).max()
+    # leading eigenvectors (well separated) against the constructed ones
+    align = np.abs(np.abs(np.einsum("kn,nk->k", Z[-64:][::-1], Q[:, :64])) - 1.0).max()
+    print("order", n, "max|w - lambda|", err_w, "residual", err_r, "orthonormality", err_o,
+          "mode 1 vs 2 (kept rows)", err_bt, "top-64 alignment", align, flush=True)
+    assert err_w <= 1e-14 * np.sqrt(n)
+    assert err_r <= 1e-12 * np.sqrt(n)
+    assert err_o <= 1e-12 * np.sqrt(n)
+    assert np.array_equal(out[1][0], out[2][0])
+    assert err_bt <= 1e-12 * np.sqrt(n)
+    assert align <= 1e-10
+
+
+def test_large_hermitian_order(engine):
+    """Complex Hermitian Gram matrix of order 1024 (TDVP bonds): the own
+    reduction + back-transformation against the constructed spectrum and
+    zheevd (mode 0)."""
+    import torch
+    n = 1024
+    rng = np.random.default_rng(77)
+    Q = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+    lam = _dmrg_like_spectrum(rng, n, keep=500)
+    A = (Q * lam) @ Q.conj().T
+    A = (A + A.conj().T) / 2
+    res = {}
+    for mode in (1, 0):
+        w, Z = engine.herm_eig(_dev(engine, np.stack([A.real, A.imag])), mode)
+        torch.cuda.synchronize()
+        w, Z = w.cpu().numpy(), Z.cpu().numpy()
+        res[mode] = (w, Z[0] + 1j * Z[1])
+    w, Z = res[1]
+    err_w = np.abs(w - np.sort(lam)).max()
+    R = A @ Z.T - Z.T * w
+    err_r = np.abs(R).max()
+    err_o = np.abs(Z.conj() @ Z.T - np.eye(n)).max()
+    err_0 = np.abs(res[1][0] - res[0][0]).max()
+    print("hermitian order", n, "max|w - lambda|", err_w, "residual", err_r, "orthonormality", err_o,
+          "vs zheevd", err_0, flush=True)
+    assert err_w <= 1e-14 * np.sqrt(n)
+    assert err_r <= 1e-12 * np.sqrt(n)
+    assert err_o <= 1e-12 * np.sqrt(n)
+    assert err_0 <= 1e-14 * np.sqrt(n)

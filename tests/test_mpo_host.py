@@ -128,3 +128,62 @@ def test_pruned_automaton_bond_dims_and_exact_zipup():
     out = om.apply_mpo(W, psi, 1e-12, 8).to_dense()
     exact = ompo.dense_matrix([2] * n, terms) @ psi.to_dense()
     assert np.linalg.norm(out - exact) < 1e-13 * np.linalg.norm(exact)
+
+
+from dmrgpy_amd.mpo import build_mpo  # noqa: E402
+
+
+def _long_range_terms(n, lam=0.5, J=1.0):
+    """sum_{i<j} J lam^(j-i-1) S_i.S_j (exponentially decaying couplings)."""
+    t = []
+    for i in range(1, n + 1):
+        for j in range(i + 1, n + 1):
+            c = J * lam ** (j - i - 1)
+            for op in ("Sx", "Sy", "Sz"):
+                t.append((complex(c), [(op, i), (op, j)]))
+    return t
+
+
+def test_compressed_mpo_long_range_is_minimal_and_exact():
+    """Exponentially decaying long-range couplings: the automaton keeps one
+    state per open S_i (w = O(N)); the SVD compression of
+    mpobuilder.py:92-103 (bidirectional truncating sweeps at the 1e-14 build
+    cutoff) reaches the minimal w = 5 with the same dense H."""
+    from dmrgpy_amd.mpo import compress_mpo, hamiltonian_mpo
+    n = 9
+    codes = [2] * n
+    terms = _long_range_terms(n)
+    A = build_mpo(codes, terms)
+    C = compress_mpo(A)
+    H = hamiltonian_mpo(codes, terms)
+    assert max(A.bond_dims()) > 5
+    assert max(C.bond_dims()) == 5
+    assert max(H.bond_dims()) == 5 and not H.complex
+    Hd = A.to_dense()
+    assert np.abs(C.to_dense() - Hd).max() < 1e-12 * np.abs(Hd).max()
+    from oracle import mpo as omp
+    ref = omp.dense_matrix(codes, terms)       # Kronecker sum of the terms (oracle)
+    assert np.abs(C.to_dense() - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_compressed_mpo_keeps_nearest_neighbour_automaton():
+    """Nearest-neighbour chains keep the (sparse, minimal) automaton MPO;
+    its compression would not lower w."""
+    from dmrgpy_amd.mpo import compress_mpo, hamiltonian_mpo
+    from dmrgpy_amd.multioperator import heisenberg_terms, hubbard_terms
+    for codes, terms, w in (([2] * 10, heisenberg_terms(10), 5), ([1] * 6, hubbard_terms(6, 1.0, 4.0), 6)):
+        H = hamiltonian_mpo(codes, terms)
+        assert H.coupling_sites and max(H.bond_dims()) == w
+        C = compress_mpo(build_mpo(codes, terms))
+        assert max(C.bond_dims()) == w
+        D = build_mpo(codes, terms).to_dense()
+        assert np.abs(C.to_dense() - D).max() < 1e-12 * np.abs(D).max()
+
+
+def test_term_gradients_refuse_compressed_mpo():
+    import pytest
+    from dmrgpy_amd.gradients import contract_term_gradients
+    from dmrgpy_amd.mpo import compress_mpo
+    C = compress_mpo(build_mpo([2] * 6, _long_range_terms(6)))
+    with pytest.raises(ValueError):
+        contract_term_gradients([w for w in C.W], C, 3)
