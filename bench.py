@@ -35,7 +35,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+# "mixed" runs BF16x3 and FP32 applies; it is priced against the FP32 peak
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "mixed": 157.3}
 METRIC = "DMRG sweep time + H_eff TFLOP/s at D=1024: mixed vs FP64, |ΔE/N|<1e-9"
 
 
@@ -229,14 +230,16 @@ def workload_heff(args):
     mpo = build_mpo([spec["code"]] * n, terms)
     eng = get_engine(0)
     precs = [args.precision] if args.only_precision else ["fp32", "fp64", "bf16x3"]
-    for pn in precs:
-        ms, flops, nbytes = heff_bulk(eng, mpo, args.D, pn, reps=max(1, args.steps))
-        mult = 3.0 if pn == "bf16x3" else 1.0
-        tf = flops / (ms * 1e-3) / 1e12
-        print(json.dumps({"workload": "H_eff apply at the bulk shape Dl=Dr={} of the {} (d={}, w={})".format(
-            args.D, spec["desc"].format(n=n), spec["d"], spec["w"]), "precision": pn, "ms_per_apply": ms,
-            "gflop_per_apply": flops / 1e9, "tflops": tf, "peak": PEAK_TFLOPS[pn],
-            "frac": mult * tf / PEAK_TFLOPS[pn], "algorithmic_bytes": nbytes, "applies": max(1, args.steps)}))
+    dims = [int(x) for x in args.heff_dims.split(",")] if args.heff_dims else [args.D]
+    for D in dims:
+        for pn in precs:
+            ms, flops, nbytes = heff_bulk(eng, mpo, D, pn, reps=max(1, args.steps))
+            mult = 3.0 if pn == "bf16x3" else 1.0
+            tf = flops / (ms * 1e-3) / 1e12
+            print(json.dumps({"workload": "H_eff apply at the bulk shape Dl=Dr={} of the {} (d={}, w={})".format(
+                D, spec["desc"].format(n=n), spec["d"], spec["w"]), "D": D, "precision": pn, "ms_per_apply": ms,
+                "gflop_per_apply": flops / 1e9, "tflops": tf, "peak": PEAK_TFLOPS[pn],
+                "frac": mult * tf / PEAK_TFLOPS[pn], "algorithmic_bytes": nbytes, "applies": max(1, args.steps)}))
 
 
 def measured_traffic(model, D, precision):
@@ -463,7 +466,7 @@ def main():
     ap.add_argument("--model", default="hubbard", choices=sorted(MODELS))
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--D", type=int, default=1024)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "mixed"])
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--svd", default="full", choices=["guided", "full"],
@@ -478,6 +481,7 @@ def main():
                     default=int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1),
                     help="BLAS threads of the all-cores CPU baseline (default: OMP_NUM_THREADS, the host "
                          "share, else os.cpu_count())")
+    ap.add_argument("--heff-dims", default="", help="--workload heff: comma-separated bond dimensions")
     ap.add_argument("--only-precision", action="store_true",
                     help="--workload heff: only the --precision given (default: fp32, fp64 and bf16x3)")
     ap.add_argument("--workload", default="dmrg", choices=["dmrg", "small", "tdvp", "gradients", "heff"],

@@ -108,10 +108,24 @@ def heff1(eng, L, R, mpo, i, dims, field, prec):
     return eng.heff_desc(L, R, mpo.W[i - 1], None, Dl, Dr, d, 0, field, prec)
 
 
+# Ritz-residual floor of the split-BF16x3 operator (the first stage of a
+# mixed-precision local solve)
+BF16X3_RES_TOL = 1e-4
+
+
 def local_ground(eng, L, R, mpo, psi, i, niter, prec, stats=None, tol=LANCZOS_TOL, res_tol=0.0):
     th, dims = theta2(eng, psi, i)
-    desc = heff2(eng, L, R, mpo, i, dims, psi.field, prec)
-    e, it = eng.lanczos(desc, th, max(niter, LANCZOS_CAP), tol, res_tol)
+    if prec == 3:
+        # mixed: BF16x3 Lanczos down to its residual floor, then FP32 Lanczos
+        # warm-started from that vector down to res_tol
+        e, it0 = eng.lanczos(heff2(eng, L, R, mpo, i, dims, psi.field, 2), th, max(niter, LANCZOS_CAP), tol,
+                             max(res_tol, BF16X3_RES_TOL))
+        e, it = eng.lanczos(heff2(eng, L, R, mpo, i, dims, psi.field, 1), th, max(niter, LANCZOS_CAP), tol,
+                            res_tol)
+        it += it0
+    else:
+        desc = heff2(eng, L, R, mpo, i, dims, psi.field, prec)
+        e, it = eng.lanczos(desc, th, max(niter, LANCZOS_CAP), tol, res_tol)
     if stats is not None:
         stats.lanczos_iters.append(it)
         stats.bond_shapes.append((i,) + tuple(dims))

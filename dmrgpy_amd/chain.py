@@ -12,8 +12,9 @@ device copy).
 
 Backend-specific settings (all optional, defaults reproduce the reference):
 
-* ``precision``: "fp64" (reference arithmetic), "fp32" or "bf16x3" for the
-  H_eff / environment contractions; Krylov vectors, the tridiagonal
+* ``precision``: "fp64" (reference arithmetic), "fp32", "bf16x3" or "mixed"
+  (each local solve in split-BF16x3 down to its residual floor, then FP32)
+  for the H_eff / environment contractions; Krylov vectors, the tridiagonal
   problems and the SVD stay FP64, and reduced-precision runs finish with an
   FP64 Rayleigh quotient <psi|H|psi>/<psi|psi>;
 * ``precision_schedule``: per-sweep precisions (e.g. reduced precision
@@ -34,7 +35,7 @@ from .mps import DeviceMPS
 BUILD_CUTOFF = 1e-14   # chain.py:38 (MPO build cutoff; the automaton MPO is exact)
 
 # reduced-precision Lanczos: relative Ritz-residual stop above the rounding floor
-RES_TOL = {1: 2e-6, 2: 1e-4}
+RES_TOL = {1: 2e-6, 2: 1e-4, 3: 2e-6}
 
 
 class Chain:

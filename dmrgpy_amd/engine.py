@@ -16,7 +16,11 @@ import numpy as np
 from . import _lib
 from ._lib import EnvDesc, HeffDesc, check
 
-PREC_NAMES = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+PREC_NAMES = {"fp64": 0, "fp32": 1, "bf16x3": 2, "mixed": 3}
+# "mixed" (host-level mode 3): FP32 environments; each local Lanczos solve
+# runs split-BF16x3 contractions to the BF16x3 residual floor, then continues
+# from that vector with FP32 contractions (algorithms.local_ground)
+MIXED = 3
 # rows of the random sketch that enlarges a guide subspace in svd_split
 SVD_SKETCH = 128
 
@@ -80,6 +84,11 @@ class Engine:
     def env_dtype(self, prec):
         return self.torch.float64 if prec == 0 else self.torch.float32
 
+    @staticmethod
+    def contraction_prec(prec):
+        """Library precision code for a host mode (mixed contracts environments in FP32)."""
+        return 1 if prec == MIXED else int(prec)
+
     def sync(self):
         check(self.lib.dmrg_sync(self.ctx), "dmrg_sync")
 
@@ -89,7 +98,7 @@ class Engine:
         d = HeffDesc()
         d.nsite = 2 if W2 is not None else 1
         d.field = field
-        d.prec = prec
+        d.prec = self.contraction_prec(prec)
         d.Dl, d.Dr, d.d1 = Dl, Dr, d1
         d.d2 = d2 if W2 is not None else 0
         W1 = np.ascontiguousarray(W1, dtype=np.float64)
@@ -163,7 +172,7 @@ class Engine:
         e = EnvDesc()
         e.dir = 0 if direction == "left" else 1
         e.field = field
-        e.prec = prec
+        e.prec = self.contraction_prec(prec)
         _, e.Dl, e.d, e.Dr = ket.shape
         _, e.Dlb, db, e.Drb = bra.shape
         if db != e.d:
