@@ -165,8 +165,13 @@ def run_sweeps(ch, psi, K, prec_name, maxdim):
     tms, flops, calls = eng.prof_read()
     nbytes = eng.prof_read_bytes()
     eng.prof_enable(False)
-    e = (alg.expectation(eng, psi, ch.H, psi) / alg.overlap(eng, psi, psi)).real
-    return {"sweep_s": dt / K, "energy": e, "heff_ms": tms[0], "heff_flops": flops[0], "heff_calls": calls[0],
+    nrm = alg.overlap(eng, psi, psi).real
+    e = (alg.expectation(eng, psi, ch.H, psi) / nrm).real
+    docc = None
+    if getattr(ch, "docc_mpo", None) is not None:
+        docc = (alg.expectation(eng, psi, ch.docc_mpo, psi) / nrm).real
+    return {"sweep_s": dt / K, "energy": e, "double_occupancy": docc,
+            "heff_ms": tms[0], "heff_flops": flops[0], "heff_calls": calls[0],
             "heff_bytes": nbytes[0],
             "env_ms": tms[1], "svd_ms": tms[2], "svd_calls": calls[2],
             "trd_ms": tms[4], "trd_calls": calls[4], "trd_bytes": nbytes[4],
@@ -243,15 +248,15 @@ def workload_heff(args):
 
 
 def measured_traffic(model, D, precision):
-    """HBM bytes per H_eff apply of this workload's timed sweep, as measured
-    with rocprofv3 FETCH_SIZE / WRITE_SIZE counters (calibrated against a
-    device copy of known size) and committed under profiles/; None when no
-    measurement of this exact workload exists."""
+    """The rocprofv3 FETCH_SIZE / WRITE_SIZE measurement of this workload's
+    timed-sweep H_eff applies committed under profiles/ (traffic and
+    algorithmic bytes per apply of the profiled applies, and the command),
+    or None when no measurement of this workload exists."""
     path = os.path.join(ROOT, "profiles", "{}_D{}_{}_heff_traffic.json".format(model, D, precision))
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)["traffic_bytes_per_apply"]
+        return json.load(f)
 
 
 def workload_tdvp(args):
@@ -524,6 +529,10 @@ def main():
     ch = Chain([spec["code"]] * n, precision=args.precision, field="real")
     ch.set_sweep_params(D, 1, 1e-12, 0.0)
     ch.set_hamiltonian(terms)
+    ch.docc_mpo = None
+    if args.model == "hubbard":   # double occupancy (1/N) sum_i <n_up n_dn>_i
+        from dmrgpy_amd.mpo import build_mpo
+        ch.docc_mpo = build_mpo(ch.site_types, [(1.0 / n, [("Nup", i), ("Ndn", i)]) for i in range(1, n + 1)])
     np.random.seed(0)
     ramp = [m for m in spec["ramp"] if m < D]
     psi = ch._default_mps(maxm=ramp[0] if ramp else D)
@@ -573,7 +582,19 @@ def main():
     # priced against the BF16 peak with the products it actually issues
     mfma_mult = 3.0 if args.precision == "bf16x3" else 1.0
     achieved = mfma_mult * flops_per / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
-    traffic = measured_traffic(args.model, D, args.precision)
+    prof = measured_traffic(args.model, D, args.precision)
+    alg_bytes = res["heff_bytes"] / max(1, res["heff_calls"])
+    traffic, traffic_note = None, None
+    if prof is not None:
+        # measured bytes per algorithmic byte of the profiled applies, applied
+        # to this run's algorithmic bytes (the profiled sweeps follow this
+        # command's trajectory when its arguments match)
+        ratio = prof["traffic_bytes_per_apply"] / prof["algorithmic_bytes_per_apply"]
+        traffic = ratio * alg_bytes
+        traffic_note = "rocprofv3 FETCH_SIZE+WRITE_SIZE of {} applies of `{}`: {:.3f} GB measured per apply, " \
+                       "{:.2f}x its algorithmic bytes; scaled to this run's algorithmic bytes".format(
+                           prof.get("applies", "?"), prof.get("profiled_command", "?"),
+                           prof["traffic_bytes_per_apply"] / 1e9, ratio)
     out = {
         "metric": METRIC,
         "value": 1.0 / res["sweep_s"],
@@ -603,8 +624,8 @@ def main():
                                    "svd": res["svd_ms"] / 1e3 / args.steps},
         "energy": res["energy"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
-                     "algorithmic_bytes": res["heff_bytes"] / max(1, res["heff_calls"]),
+                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_note,
+                     "algorithmic_bytes": alg_bytes,
                      "kernel": "H_eff two-site apply (L-GEMM, W small-op, R-GEMM), {:.3g} GFLOP per apply "
                                "averaged over the sweep's bonds{}, {:.3f} ms average".format(
                                    flops_per / 1e9, " (x3 BF16 products)" if mfma_mult > 1 else "", avg_ms)},
@@ -650,6 +671,15 @@ def main():
                                                   "svd": res64["svd_ms"] / 1e3 / args.steps}}
         out["speedup_vs_fp64"] = res64["sweep_s"] / res["sweep_s"]
         out["dE_per_site_vs_fp64"] = abs(res["energy"] - res64["energy"]) / n
+        # observables of the final states against FP64 (truncation weight:
+        # summed discarded weight of the last timed sweep's 2(N-1) splits)
+        out["observables_vs_fp64"] = {
+            "energy": [res["energy"], res64["energy"]],
+            "double_occupancy": [res["double_occupancy"], res64["double_occupancy"]],
+            "truncation_weight": [res["truncation_weight"], res64["truncation_weight"]]}
+        if res["double_occupancy"] is not None:
+            out["observables_vs_fp64"]["d_double_occupancy"] = abs(res["double_occupancy"] -
+                                                                   res64["double_occupancy"])
     if cpu is not None:
         out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["cpu_baseline_one_thread"] = cpu["one_thread"]

@@ -74,8 +74,25 @@ __device__ __forceinline__ double sum_one(const double* part, int nb, double* re
   return r;
 }
 
+// Unroll depth of the loops that stream the trailing matrix (launch 2) and
+// the previous launch's partial sums: each unrolled batch keeps that many
+// independent loads in flight per thread.  Deeper unrolling was measured no
+// faster (sym_eig at order 3000: 61.8 / 63.2 / 67.8 ms with 4 / 8 / 16; 4
+// rows per workgroup in launch 2 instead of 2: 63.8 ms)
+#ifndef TRD_UNROLL_STREAM
+#define TRD_UNROLL_STREAM 4
+#endif
+#ifndef TRD_UNROLL_PART
+#define TRD_UNROLL_PART 4
+#endif
+#define TRD_STR(x) #x
+#define TRD_UNROLL(n) _Pragma(TRD_STR(unroll n))
+
 constexpr int TRD_ROWS1 = 64;   // rows per workgroup in launch 1 (32 / 64 / 128 measured; 64 best at orders >= 2000)
-constexpr int TRD_ROWS2 = 2;    // rows per workgroup in launch 2 (measured best of 1, 2, 4, 8, 16)
+#ifndef TRD_ROWS2_CFG
+#define TRD_ROWS2_CFG 2
+#endif
+constexpr int TRD_ROWS2 = TRD_ROWS2_CFG;    // rows per workgroup in launch 2 (measured best of 1, 2, 4, 8, 16)
 
 // Both launches issue every global load they need up front (all of them
 // miss the caches: the previous launch wrote the data, possibly from another
@@ -97,7 +114,7 @@ __global__ __launch_bounds__(256) void trd_col_k(const double* __restrict__ A, i
   const double tprev = st->t;
   double ps = 0.0;   // this thread's share of column j-1's w^T v partials
   if (jj > 0) {
-#pragma unroll 4
+TRD_UNROLL(TRD_UNROLL_PART)
     for (int b = tid; b < g2prev; b += 256) ps += part2[b];
   }
   // panel rows into registers first (all loads in flight at once), then LDS
@@ -202,7 +219,7 @@ __global__ __launch_bounds__(256) void trd_symv_k(double* __restrict__ A, int n,
   {
     const int k = tid & 63, grp = tid >> 6;
     if (k < nk) {
-#pragma unroll 4
+TRD_UNROLL(TRD_UNROLL_PART)
       for (int b = grp; b < g1; b += 4) ps += part1[(size_t)b * TRD_PS + k];
     }
   }
@@ -232,7 +249,7 @@ __global__ __launch_bounds__(256) void trd_symv_k(double* __restrict__ A, int n,
   double acc[TRD_ROWS2], a1[TRD_ROWS2];
 #pragma unroll
   for (int i = 0; i < TRD_ROWS2; ++i) acc[i] = a1[i] = 0.0;
-#pragma unroll 4
+TRD_UNROLL(TRD_UNROLL_STREAM)
   for (int c = j + 1 + tid; c < n; c += 256) {
     const bool first = c == j + 1;
     const double v = first ? 0.0 : avec[c];

@@ -167,6 +167,8 @@ def main():
     ap.add_argument("--cal-fetch")
     ap.add_argument("--cal-write")
     ap.add_argument("--cal-bytes", type=float, default=float(1 << 30))
+    ap.add_argument("--mfma", help="counter CSV of a --pmc MfmaUtil run: duration-weighted MFMA utilisation "
+                                   "(percent of the SIMDs' cycles busy in MFMA) per role")
     a = ap.parse_args()
     if a.stats:
         print(kernel_stats(a.stats, a.top))
@@ -191,6 +193,16 @@ def main():
         res["fetch_bytes"] = {k: f[k] * kf for k in ROLES if k in f}
         res["write_bytes"] = {k: w[k] * kw for k in ROLES if k in w}
         res["traffic_bytes_per_apply"] = sum(res["fetch_bytes"].values()) + sum(res["write_bytes"].values())
+    if a.mfma:
+        ma = group_applies(read_dispatches(a.mfma, "MfmaUtil"))[-a.applies:]
+        num, den = {}, {}
+        for grp in ma:
+            for d in grp:
+                num[d[1]] = num.get(d[1], 0.0) + d[4] * d[3]
+                den[d[1]] = den.get(d[1], 0.0) + d[3]
+        res["mfma_util_percent"] = {k: num[k] / den[k] for k in ROLES if k in num and den[k] > 0}
+        tot_n, tot_d = sum(num.values()), sum(den.values())
+        res["mfma_util_percent_apply"] = tot_n / tot_d if tot_d > 0 else None
     print(json.dumps(res, indent=1))
 
 
