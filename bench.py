@@ -570,6 +570,14 @@ def main():
         res64 = run_sweeps(ch, psi64, args.steps, "fp64", D)
         log("timed fp64: {}".format({k: v for k, v in res64.items() if k not in ("shapes", "iters")}))
 
+    # the H_eff apply at the bulk shape Dl = Dr = D itself (the sweep's bonds
+    # stay below D at the reference cutoff): the same kernels on the model's
+    # middle MPO tensors, HIP events on the engine stream
+    bulk_prec = "fp32" if args.precision == "mixed" else args.precision
+    bulk_ms, bulk_flops, bulk_bytes = heff_bulk(eng, ch.H, D, bulk_prec, reps=10)
+    bulk_mult = 3.0 if bulk_prec == "bf16x3" else 1.0
+    bulk_tf = bulk_flops / (bulk_ms * 1e-3) / 1e12
+    log("bulk H_eff at D={}: {:.3f} ms, {:.1f} TFLOP/s".format(D, bulk_ms, bulk_tf))
     heff_tflops = res["heff_flops"] / (res["heff_ms"] * 1e-3) / 1e12 if res["heff_ms"] > 0 else 0.0
     avg_ms = res["heff_ms"] / max(1, res["heff_calls"])
     flops_per = res["heff_flops"] / max(1, res["heff_calls"])
@@ -630,6 +638,11 @@ def main():
                                "averaged over the sweep's bonds{}, {:.3f} ms average".format(
                                    flops_per / 1e9, " (x3 BF16 products)" if mfma_mult > 1 else "", avg_ms)},
     }
+    out["heff_bulk"] = {"D": D, "precision": bulk_prec, "ms_per_apply": bulk_ms, "gflop_per_apply": bulk_flops / 1e9,
+                        "tflops": bulk_tf, "peak": PEAK_TFLOPS[bulk_prec],
+                        "frac": bulk_mult * bulk_tf / PEAK_TFLOPS[bulk_prec],
+                        "kernel": "H_eff two-site apply at Dl = Dr = {} with the model's bulk MPO tensors and random "
+                                  "environments, 10 applies".format(D)}
     if res["trd_calls"] > 0 and res["trd_ms"] > 0:
         # the Gram eigensolver's tridiagonalisation, the largest cost of a sweep:
         # algorithmic bytes = the lower triangle of the trailing matrix once per column
