@@ -305,9 +305,10 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
 int dmrg_set_eigensolver(dmrg_ctx* ctx, int mode) {
   API_BEGIN
   DMRG_REQUIRE(ctx, "set_eigensolver: null context");
-  DMRG_REQUIRE(mode >= 0 && mode <= 2,
-               "set_eigensolver: mode must be 0 (rocSOLVER syevd), 1 (own tridiagonalisation and back-"
-               "transformation) or 2 (own tridiagonalisation, rocSOLVER ormtr)");
+  DMRG_REQUIRE(mode >= 0 && mode <= 3,
+               "set_eigensolver: mode must be 0 (rocSOLVER syevd), 1 (own tridiagonalisation with early stop "
+               "and back-transformation), 2 (own tridiagonalisation, rocSOLVER ormtr) or 3 (as 1, the "
+               "reduction always complete)");
   ctx->c.eig_mode = mode;
   API_END
 }

@@ -75,7 +75,7 @@ enum Slot {
   WS_SVD_GRAM, WS_SVD_EV, WS_SVD_TMP, WS_SVD_TMP2, WS_SVD_INFO, WS_SVD_CPLX, WS_SVD_FIX, WS_SVD_RED, WS_PROJ, WS_B3_L, WS_B3_R,
   WS_BF16_A, WS_BF16_B, WS_REFINE,
   WS_GSV_Q, WS_GSV_B, WS_GSV_R, WS_GSV_S, WS_GSV_V, WS_GSV_T, WS_GSV_T2, WS_GSV_OU, WS_GSV_OV,
-  WS_TRD, WS_TRD_Z, WS_TRD_ZC, WS_BT_V, WS_BT_W,
+  WS_TRD, WS_TRD_Z, WS_TRD_ZC, WS_BT_V, WS_BT_W, WS_TRD_ZS,
   WS_NSLOTS
 };
 
@@ -103,10 +103,12 @@ struct Ctx {
   long long prof_calls[5] = {0, 0, 0, 0, 0};
   long long matvecs = 0;
   // dense eigensolver of the Gram route: 1 (default) = own two-launch
-  // tridiagonalisation (real or complex Hermitian) + rocSOLVER stedc + the
-  // library's blocked back-transformation of the kept eigenvectors; 2 = the
-  // same with rocSOLVER ormtr / unmtr for the back-transformation;
-  // 0 = rocSOLVER syevd / heevd
+  // tridiagonalisation (real or complex Hermitian; a real reduction stops
+  // once the trailing block is below TRD_STOP_TOL |G|_F) + rocSOLVER stedc +
+  // the library's blocked back-transformation of the kept eigenvectors;
+  // 3 = the same with the reduction always complete; 2 = complete reduction
+  // with rocSOLVER ormtr / unmtr for the back-transformation; 0 = rocSOLVER
+  // syevd / heevd
   int eig_mode = 1;
   // allocation-free calls (dmrg_heff_apply_noalloc): while `noalloc` is set,
   // ws_get carves the workspace out of this device's static scratch array
@@ -120,6 +122,12 @@ struct Ctx {
 };
 
 void* ws_get(Ctx* c, Slot s, size_t bytes);
+
+// Early stop of the Gram matrices' tridiagonalisation (eigensolver mode 1):
+// relative Frobenius norm below which the trailing block is taken as zero
+// (about one rounding of the matrix; every eigenvalue moves by at most this
+// times |G|_F)
+constexpr double TRD_STOP_TOL = 2e-16;
 
 // ---- GEMM (gemm.hip) ------------------------------------------------------
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b], row-major, logical A is

@@ -61,16 +61,19 @@ void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, 
                  int inner, long long sx_in, long long sx_out, long long pin, long long pout);
 void gemm_prec(Ctx* c, int prec, const GemmDesc& g, const void* A, const void* B, void* C);
 bool smallop_sparse_ok(const SmallOp& s);
-// dsytrd (uplo 'L') of a symmetric n x n matrix in place: d[n], e[n-1], tau[n-1]
-void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau);
+// dsytrd (uplo 'L') of a symmetric n x n matrix in place: d[n], e[n-1],
+// tau[n-1]; returns the number of reflectors computed (n - 1, or fewer when
+// stop_tol > 0 and the trailing block fell below stop_tol |A|_F: the rest of
+// d, e, tau is then zero)
+int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol = 0.0);
 // zhetrd (uplo 'L') of the column-major view of a Hermitian n x n planar
 // complex buffer (plane stride n*n) in place: d[n], e[n-1] real, tau[n-1]
 // complex interleaved
 void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau);
 // rows of Zt (keep x n, row-major: eigenvectors of the tridiagonal) <- rows
-// of Q Y for the reflectors sytrd_lower left in A (dormtr L/lower/N by
-// blocked compact-WY GEMMs)
-void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Zt, int keep);
+// of Q Y for the first nrefl reflectors sytrd_lower left in A (all n - 1 by
+// default; dormtr L/lower/N by blocked compact-WY GEMMs)
+void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Zt, int keep, int nrefl = -1);
 // complex: rows of Et (keep x n, planar, plane stride keep*n) <- rows of Q Y
 // for hetrd_lower's reflectors (zunmtr L/lower/N)
 void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Et, int keep);

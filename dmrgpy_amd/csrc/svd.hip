@@ -121,6 +121,16 @@ __global__ void copy_block_k(const double* __restrict__ src, int lds, int rows, 
   }
 }
 
+// dst[r][c] = src[r][c] for c < m, 0 for m <= c < g (keep rows): the kept
+// eigenvectors of a leading order-m tridiagonal block as rows of length g
+__global__ void embed_rows_k(const double* __restrict__ src, int m, int keep, int g, double* __restrict__ dst) {
+  const long long tot = (long long)keep * g;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(i / g), cc = (int)(i % g);
+    dst[i] = cc < m ? src[(size_t)r * m + cc] : 0.0;
+  }
+}
+
 // Two-pass max reductions (the maximum is order independent, so the result
 // is deterministic): block maxima into part[], then one block over part[].
 // mode 0: max |x_i| over n elements; mode 1: max |T - I| over a k x k planar
@@ -432,7 +442,8 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
   double* amax_dev = ev + 2 * (size_t)g + 8;
   rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
   double* Gsolve = nullptr;
-  double *trd_Z = nullptr, *trd_tau = nullptr, *trd_refl = nullptr;
+  double *trd_Z = nullptr, *trd_tau = nullptr, *trd_refl = nullptr, *trd_Zs = nullptr;
+  int trd_nrefl = g - 1, trd_m = g;   // reflectors computed, order of the tridiagonal solved
   {
     ProfScope ps(c, 2, (cx ? 8.0 : 2.0) * (double)g * g * (left ? n : m));
     double* Gplanar = (double*)ws_get(c, WS_SVD_TMP2, sizeof(double) * es * (size_t)g * g);
@@ -468,11 +479,22 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
       Gsolve = Gi;
     } else if (c->eig_mode >= 1 && g >= 32) {
       // tridiagonalise here, eigenpairs of the tridiagonal by rocSOLVER's
-      // divide and conquer; the back-transformation waits for keep
+      // divide and conquer; the back-transformation waits for keep.  In
+      // mode 1 the reduction stops once the trailing block is below
+      // TRD_STOP_TOL |G|_F (sytrd_lower), and divide and conquer then runs
+      // on the leading order-(nrefl + 1) tridiagonal only; the rest of the
+      // spectrum is exactly zero.
       trd_Z = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * ((size_t)g * g + g + 16));
       trd_tau = trd_Z + (size_t)g * g;
-      sytrd_lower(c, Gplanar, g, ev, ework, trd_tau);
-      check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, g, ev, ework, trd_Z, g, info), "dstedc");
+      trd_nrefl = sytrd_lower(c, Gplanar, g, ev, ework, trd_tau, c->eig_mode == 1 ? TRD_STOP_TOL : 0.0);
+      trd_m = trd_nrefl < g - 1 ? trd_nrefl + 1 : g;
+      if (trd_m < g) {
+        trd_Zs = (double*)ws_get(c, WS_TRD_ZS, sizeof(double) * (size_t)trd_m * trd_m);
+        check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, trd_m, ev, ework, trd_Zs, trd_m, info),
+                 "dstedc");
+      } else {
+        check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, g, ev, ework, trd_Z, g, info), "dstedc");
+      }
       Gsolve = trd_Z;
       trd_refl = Gplanar;
     } else {
@@ -481,10 +503,13 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
       Gsolve = Gplanar;
     }
   }
-  std::vector<double> evh(g);
+  std::vector<double> evh(g, 0.0);
   int hinfo = 0;
   double hmax = 0.0;
-  DMRG_CHECK_HIP(hipMemcpyAsync(evh.data(), ev, sizeof(double) * g, hipMemcpyDeviceToHost, c->stream));
+  // ascending; a reduction that stopped early leaves g - trd_m exact zeros
+  // below the trd_m computed values
+  DMRG_CHECK_HIP(hipMemcpyAsync(evh.data() + (g - trd_m), ev, sizeof(double) * trd_m, hipMemcpyDeviceToHost,
+                                c->stream));
   DMRG_CHECK_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   DMRG_CHECK_HIP(hipMemcpyAsync(&hmax, amax_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
@@ -495,15 +520,21 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
   SvdOut o;
   o.keep = truncate_rule(svals, cutoff, maxdim, mind, &o.discarded, extra);
   const int keep = o.keep;
-  if (!(svals[0] > 0.0 && svals[keep - 1] >= 1e-7 * svals[0]))
+  if (!(svals[0] > 0.0 && svals[keep - 1] >= 1e-7 * svals[0]) || keep > trd_m)
     return svd_full(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals);
   DMRG_REQUIRE(keep <= kcap, "svd: keep exceeds output capacity");
   if (trd_refl && !cx) {
     // eigenvectors of G for the kept (largest) values only: Q times the
     // last keep eigenvector columns of the tridiagonal
-    ProfScope ps(c, 2, 2.0 * (double)g * g * keep);
-    if (c->eig_mode == 1)
-      ormtr_lower_rows(c, trd_refl, g, trd_tau, trd_Z + (size_t)(g - keep) * g, keep);
+    ProfScope ps(c, 2, 2.0 * (double)g * trd_nrefl * keep);
+    double* Zt = trd_Z + (size_t)(g - keep) * g;
+    if (trd_m < g) {   // the kept eigenvectors of the leading block, zero beyond it
+      const long long tot = (long long)keep * g;
+      embed_rows_k<<<g1(tot), 256, 0, c->stream>>>(trd_Zs + (size_t)(trd_m - keep) * trd_m, trd_m, keep, g, Zt);
+      DMRG_CHECK_HIP(hipGetLastError());
+    }
+    if (c->eig_mode != 2)
+      ormtr_lower_rows(c, trd_refl, g, trd_tau, Zt, keep, trd_nrefl);
     else
       check_rb(rocsolver_dormtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, g, keep,
                                 trd_refl, g, trd_tau, trd_Z + (size_t)(g - keep) * g, g),
@@ -515,7 +546,7 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
     double* Zc = (double*)ws_get(c, WS_TRD_ZC, sizeof(double) * 2 * (size_t)g * g);
     double* Zk = Zc + 2 * (size_t)(g - keep) * g;
     const long long kg = (long long)keep * g;
-    if (c->eig_mode == 1) {   // own blocked product on planar rows
+    if (c->eig_mode != 2) {   // own blocked product on planar rows
       double* Et = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)kg);
       DMRG_CHECK_HIP(hipMemcpyAsync(Et, trd_Z + (size_t)(g - keep) * g, sizeof(double) * kg,
                                     hipMemcpyDeviceToDevice, c->stream));
@@ -908,17 +939,18 @@ int svd_split_entry(Ctx* c, int field, const double* M, int m, int n, double cut
 
 // All eigenpairs of a real symmetric matrix (row-major = column-major):
 // w ascending, Z[k][:] the eigenvector of w[k] (rows of the row-major Z).
-// mode 1: own tridiagonalisation + stedc + own back-transformation, mode 2:
-// the same with rocSOLVER's ormtr, mode 0: syevd.  A is overwritten.
+// modes 1 and 3: own tridiagonalisation (always complete here) + stedc + own
+// back-transformation, mode 2: the same with rocSOLVER's ormtr, mode 0:
+// syevd.  A is overwritten.
 int sym_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
   DMRG_REQUIRE(n >= 2, "sym_eig: order must be at least 2");
   double* ework = (double*)ws_get(c, WS_SVD_EV, sizeof(double) * ((size_t)2 * n + 16));
   double* tauv = ework + n;
   rocblas_int* info = (rocblas_int*)ws_get(c, WS_SVD_INFO, 64);
-  if (mode == 1 || mode == 2) {
+  if (mode >= 1) {   // every eigenvector is wanted: always the full reduction
     sytrd_lower(c, A, n, w, ework, tauv);
     check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, n, w, ework, Z, n, info), "dstedc");
-    if (mode == 1)
+    if (mode != 2)
       ormtr_lower_rows(c, A, n, tauv, Z, n);
     else
       check_rb(rocsolver_dormtr(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, A, n,
@@ -938,8 +970,8 @@ int sym_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
 
 // All eigenpairs of a Hermitian matrix A (row-major, planar complex, plane
 // stride n*n): w ascending, Z planar with Z[k][:] the eigenvector of w[k].
-// mode 1: own reduction + stedc + own back-transformation, mode 2: the same
-// with rocSOLVER's unmtr, mode 0: heevd.  A is overwritten.
+// modes 1 and 3: own reduction + stedc + own back-transformation, mode 2:
+// the same with rocSOLVER's unmtr, mode 0: heevd.  A is overwritten.
 // The column-major view of the buffer is conj(A); its eigenvectors come out
 // as rows of the interleaved buffer and are conjugated on the way out.
 int herm_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
@@ -954,7 +986,7 @@ int herm_eig_entry(Ctx* c, int mode, double* A, int n, double* w, double* Z) {
     double* Zr = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * (size_t)nn);
     hetrd_lower(c, A, n, w, ework, tauv);
     check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, n, w, ework, Zr, n, info), "dstedc");
-    if (mode == 1) {   // Ai holds the planar rows of Q Y
+    if (mode != 2) {   // Ai holds the planar rows of Q Y
       DMRG_CHECK_HIP(hipMemcpyAsync(Ai, Zr, sizeof(double) * nn, hipMemcpyDeviceToDevice, c->stream));
       DMRG_CHECK_HIP(hipMemsetAsync(Ai + nn, 0, sizeof(double) * nn, c->stream));
       unmtr_lower_rows(c, A, n, tauv, Ai, n);

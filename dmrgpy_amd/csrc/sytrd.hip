@@ -332,9 +332,69 @@ __global__ __launch_bounds__(256) void trd_fin_k(int n, int s, int k, const doub
   if (r < n) W[(size_t)r * TRD_P + k] = wpre[r] + alpha * V[(size_t)r * TRD_P + k];
 }
 
+// Sum of squares of the trailing block A[s:, s:] (pitch n): per-block
+// partial sums over whole rows, then one fixed-order sum (deterministic).
+constexpr int TRD_SQ_BLOCKS = 512;
+__global__ __launch_bounds__(256) void trd_sumsq_k(const double* __restrict__ A, int n, int s,
+                                                   double* __restrict__ part) {
+  __shared__ double red4[4];
+  double acc = 0.0;
+  for (int r = s + blockIdx.x; r < n; r += gridDim.x) {
+    const double* row = A + (size_t)r * n;
+    for (int cc = s + threadIdx.x; cc < n; cc += 256) {
+      const double x = row[cc];
+      acc += x * x;
+    }
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red4[0] + red4[1] + red4[2] + red4[3];
+}
+
+__global__ __launch_bounds__(256) void trd_sumpart_k(const double* __restrict__ part, int nb,
+                                                     double* __restrict__ out) {
+  __shared__ double red4[256];
+  const double v = sum_one(part, nb, red4);
+  if (threadIdx.x == 0) *out = v;
+}
+
+// d[s..n-1] = 0, e[s..n-2] = 0, tau[s..n-2] = 0: the tridiagonal of a
+// reduction stopped after s columns (the trailing block taken as zero)
+__global__ void trd_clear_tail_k(int n, int s, double* __restrict__ d, double* __restrict__ e,
+                                 double* __restrict__ tau) {
+  for (int i = s + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    d[i] = 0.0;
+    if (i < n - 1) {
+      e[i] = 0.0;
+      tau[i] = 0.0;
+    }
+  }
+}
+
+static double trailing_sumsq(Ctx* c, const double* A, int n, int s, double* part, int cap, double* out) {
+  const int nb = std::max(1, std::min(std::min(TRD_SQ_BLOCKS, n - s), cap));
+  trd_sumsq_k<<<nb, 256, 0, c->stream>>>(A, n, s, part);
+  DMRG_CHECK_HIP(hipGetLastError());
+  trd_sumpart_k<<<1, 256, 0, c->stream>>>(part, nb, out);
+  DMRG_CHECK_HIP(hipGetLastError());
+  DMRG_CHECK_HIP(hipMemcpyAsync(c->host_scalars + 32, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return c->host_scalars[32];
+}
+
 // A (n x n, symmetric, row-major = column-major) -> d[n], e[n-1], tau[n-1]
-// and reflectors in place (dsytrd, uplo 'L').
-void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
+// and reflectors in place (dsytrd, uplo 'L').  Returns the number of
+// reflectors computed: n - 1, or, with stop_tol > 0, possibly fewer -- after
+// each panel (from column TRD_STOP_MIN on) the Frobenius norm of the
+// trailing block is compared with stop_tol |A|_F, and once it is below, the
+// block is taken as zero: the remaining d, e and tau are set to 0 and the
+// tridiagonal is complete after s columns.  That perturbs A by at most
+// stop_tol |A|_F (every eigenvalue by at most as much), far below the
+// rounding of the reduction itself at stop_tol ~ 2e-16; the dense
+// eigenproblem then shrinks to order s + 1 plus zeros.
+constexpr int TRD_STOP_MIN = 256;
+int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol) {
   DMRG_REQUIRE(n >= 2, "sytrd: order must be at least 2");
   const size_t vwv = (size_t)n * TRD_P;   // the [V | W | V] panel store
   const int gmax1 = (n + TRD_ROWS1 - 1) / TRD_ROWS1;
@@ -350,10 +410,15 @@ void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
   double* part2 = part1 + (size_t)gmax1 * TRD_PS;
   TrdState* st = reinterpret_cast<TrdState*>(part2 + gmax2 + 8);
   DMRG_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(TrdState), c->stream));
-  // algorithmic bytes: the lower triangle of the trailing matrix, once per column
-  double bytes = 0.0;
-  for (int j = 0; j < n - 1; ++j) bytes += 8.0 * (double)(n - j - 1) * (n - j) / 2.0;
-  ProfScope ps(c, 4, 0.0, bytes);
+  ProfScope ps(c, 4, 0.0, 0.0);
+  // early stop: |A|_F of the input, and scratch for the trailing sums (the
+  // launch-1 partial buffer is free between panels)
+  const bool early = stop_tol > 0.0 && n > TRD_STOP_MIN + TRD_NB;
+  double* sq_out = part2 + gmax2;    // 8 spare doubles before the state
+  double stop2 = 0.0;
+  const int sq_cap = gmax1 * TRD_PS;   // partial sums the launch-1 buffer holds
+  if (early) stop2 = stop_tol * stop_tol * trailing_sumsq(c, A, n, 0, part1, sq_cap, sq_out);
+  int done = n - 1;   // reflectors computed
   int g2prev = 1;
   for (int j0 = 0; j0 < n - 1; j0 += TRD_NB) {
     const int nbp = std::min(TRD_NB, n - 1 - j0);
@@ -389,10 +454,21 @@ void sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
         gemm_f64(c, g, Vs, Vs + TRD_NB, C);
         gemm_f64(c, g, Vs + TRD_NB, Vs, C);
       }
+      if (early && nbp == TRD_NB && s >= TRD_STOP_MIN && s < n - 1 &&
+          trailing_sumsq(c, A, n, s, part1, sq_cap, sq_out) <= stop2) {
+        trd_clear_tail_k<<<(n - s + 255) / 256, 256, 0, c->stream>>>(n, s, d, e, tau);
+        DMRG_CHECK_HIP(hipGetLastError());
+        done = s;
+        break;
+      }
     }
   }
-  DMRG_CHECK_HIP(hipMemcpyAsync(d + (n - 1), A + (size_t)(n - 1) * n + (n - 1), sizeof(double),
-                                hipMemcpyDeviceToDevice, c->stream));
+  if (done == n - 1)
+    DMRG_CHECK_HIP(hipMemcpyAsync(d + (n - 1), A + (size_t)(n - 1) * n + (n - 1), sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
+  // algorithmic bytes: the lower triangle of the trailing matrix, once per reduced column
+  for (int j = 0; j < done; ++j) ps.bytes += 8.0 * (double)(n - j - 1) * (n - j) / 2.0;
+  return done;
 }
 
 
@@ -860,11 +936,12 @@ constexpr int BT_SPLIT = 8;
 
 // Vall[j][r] = v_j[r] for rows j < n - 1 (0 for r <= j, 1 at r = j + 1, the
 // stored entry A[j][r] beyond), zero rows up to the block multiple
-__global__ void bt_vall_k(const double* __restrict__ A, int n, long long total, double* __restrict__ Vall) {
+__global__ void bt_vall_k(const double* __restrict__ A, int n, int nrefl, long long total,
+                          double* __restrict__ Vall) {
   for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long long)gridDim.x * 256) {
     const int j = (int)(idx / n), r = (int)(idx % n);
     double v = 0.0;
-    if (j < n - 1) {
+    if (j < nrefl) {
       if (r == j + 1) v = 1.0;
       else if (r > j + 1) v = A[(size_t)j * n + r];
     }
@@ -882,14 +959,14 @@ __device__ __forceinline__ int bt_s_idx(int l, int i) { return i * (i - 1) / 2 +
 __device__ __forceinline__ int bt_t_idx(int k, int l) { return k * BT_NB - k * (k - 1) / 2 + (l - k); }  // l >= k
 
 __global__ __launch_bounds__(BT_NB) void bt_larft_k(const double* __restrict__ S, const double* __restrict__ tau,
-                                                   int n, double* __restrict__ TtS) {
+                                                   int nrefl, double* __restrict__ TtS) {
   __shared__ double sS[BT_TRI];
   __shared__ double sT[BT_TRI];
   __shared__ double stau[BT_NB];
   const int b = blockIdx.x, k = threadIdx.x;
   const int j0 = b * BT_NB;
   const double* Sb = S + (size_t)b * BT_NB * BT_NB;
-  stau[k] = j0 + k < n - 1 ? tau[j0 + k] : 0.0;
+  stau[k] = j0 + k < nrefl ? tau[j0 + k] : 0.0;
 #pragma unroll 16
   for (int t = 0; t < BT_NB; ++t) {   // S[t][k], kept for t < k (column k of the strict upper triangle)
     const double v = Sb[(size_t)t * BT_NB + k];
@@ -940,9 +1017,10 @@ __global__ void bt_sum_k(const double* __restrict__ Ycat, int rows, int nb, int 
 // Zt (keep x n, row-major, the rows eigenvectors of the tridiagonal) <- rows
 // of Q Y, Q from the reflectors in A (row-major n x n as sytrd_lower leaves
 // it) and tau
-void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Zt, int keep) {
-  DMRG_REQUIRE(n >= 2 && keep >= 1 && keep <= n, "ormtr: bad sizes");
-  const int nblk = (n - 1 + BT_NB - 1) / BT_NB;
+void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Zt, int keep, int nrefl) {
+  if (nrefl < 0) nrefl = n - 1;
+  DMRG_REQUIRE(n >= 2 && keep >= 1 && keep <= n && nrefl >= 1 && nrefl <= n - 1, "ormtr: bad sizes");
+  const int nblk = (nrefl + BT_NB - 1) / BT_NB;
   const size_t vrows = (size_t)nblk * BT_NB;
   double* Vall = (double*)ws_get(c, WS_BT_V, sizeof(double) * vrows * n);
   const size_t nS = (size_t)nblk * BT_NB * BT_NB, nT = (size_t)nblk * BT_SPLIT * BT_NB * BT_NB;
@@ -952,7 +1030,7 @@ void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
   double* Ycat = TtS + nT;
   double* Y2 = Ycat + nY;
   const long long total = (long long)vrows * n;
-  bt_vall_k<<<(int)std::min<long long>((total + 255) / 256, 65536), 256, 0, c->stream>>>(A, n, total, Vall);
+  bt_vall_k<<<(int)std::min<long long>((total + 255) / 256, 65536), 256, 0, c->stream>>>(A, n, nrefl, total, Vall);
   DMRG_CHECK_HIP(hipGetLastError());
   {  // S_b = V_b V_b^T for every block at once
     GemmDesc g = gdesc(BT_NB, BT_NB, n, n, n, BT_NB, REAL, 4);
@@ -962,7 +1040,7 @@ void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
     g.sC = (long long)BT_NB * BT_NB;
     gemm_f64(c, g, Vall, Vall, Sm);
   }
-  bt_larft_k<<<nblk, BT_NB, 0, c->stream>>>(Sm, tau, n, TtS);
+  bt_larft_k<<<nblk, BT_NB, 0, c->stream>>>(Sm, tau, nrefl, TtS);
   DMRG_CHECK_HIP(hipGetLastError());
   for (int b = nblk - 1; b >= 0; --b) {
     const int b0 = b * BT_NB;

@@ -229,11 +229,13 @@ class Engine:
         return U, V, k, disc.value, sv
 
     def set_eigensolver(self, mode):
-        """1 (default): own tridiagonalisation (real or Hermitian) +
-        rocSOLVER stedc + the library's blocked back-transformation of the
-        kept eigenvectors; 2: as 1 with rocSOLVER ormtr/unmtr for the
-        back-transformation; 0: rocSOLVER syevd/heevd -- for the Gram-route
-        splits."""
+        """Eigensolver of the Gram-route splits.  1 (default): own
+        tridiagonalisation (real or Hermitian) + rocSOLVER stedc + the
+        library's blocked back-transformation of the kept eigenvectors, the
+        real reduction stopping once its trailing block is below 2e-16 of
+        the Gram matrix in Frobenius norm (orders above 288); 3: as 1 with
+        the complete reduction; 2: as 3 with rocSOLVER ormtr/unmtr for the
+        back-transformation; 0: rocSOLVER syevd/heevd."""
         check(self.lib.dmrg_set_eigensolver(self.ctx, int(mode)), "dmrg_set_eigensolver")
 
     def herm_eig(self, A, mode=1):

@@ -400,3 +400,99 @@ def test_large_hermitian_order(engine):
     assert err_r <= 1e-12 * np.sqrt(n)
     assert err_o <= 1e-12 * np.sqrt(n)
     assert err_0 <= 1e-14 * np.sqrt(n)
+
+
+# ---- early stop of the reduction (mode 1) against the complete reduction
+# (mode 3): the split must keep the same count and discard the same weight
+
+@pytest.mark.parametrize("shape", [(2400, 2560), (1800, 1700), (330, 340)])
+def test_split_early_stop_matches_full_reduction(engine, shape):
+    """A graded two-site-like matrix M = diag(r) Z diag(c) (Z Gaussian; r, c
+    falling by 1e-10 across the rows / columns, as Schmidt weights of the
+    neighbouring bonds grade a DMRG theta), so the Gram matrix's rounding
+    noise is graded too and its trailing blocks fall far below eps |G|_F:
+    the reduction of mode 1 stops once the trailing block is below 2e-16 of
+    |G|_F (at about half the order here, CPU Householder); the split must
+    then agree with the complete reduction (mode 3) -- keep count,
+    discarded weight to 1e-14, kept singular values and the truncated
+    product within the Gram-route rounding bound, isometry -- and must have
+    reduced fewer columns (algorithmic bytes of the tridiagonalisation)."""
+    import time
+    import torch
+    rng = np.random.default_rng(shape[0])
+    m, n = shape
+    r = 10.0 ** (-10.0 * np.arange(m) / m)
+    c = 10.0 ** (-10.0 * np.arange(n) / n)
+    M = (r[:, None] * rng.standard_normal((m, n))) * c[None, :]
+    Md = _dev(engine, M[None])
+    out = {}
+    try:
+        for mode in (3, 1):
+            engine.set_eigensolver(mode)
+            engine.svd_split(0, Md, m, n, 1e-12, 0, 1, 0)     # warm-up (workspace)
+            engine.prof_enable(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            U, V, keep, disc, sv = engine.svd_split(0, Md, m, n, 1e-12, 0, 1, 0)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            trd_bytes = engine.prof_read_bytes()[4]
+            engine.prof_enable(False)
+            out[mode] = (U.cpu().numpy()[0], V.cpu().numpy()[0], keep, disc, sv, dt, trd_bytes)
+    finally:
+        engine.set_eigensolver(1)   # the default
+    Uf, Vf, kf, df, sf, tf, bf = out[3]
+    Ue, Ve, ke, de, se, te, be = out[1]
+    print("shape", shape, "keep", kf, ke, "discarded", df, de, "split ms full {:.1f} early {:.1f}".format(
+        1e3 * tf, 1e3 * te), "reduction bytes ratio", be / bf, flush=True)
+    assert ke == kf
+    assert abs(de - df) < 1e-14
+    eps = 2.2e-16
+    assert np.all(np.abs(se[:ke] - sf[:kf]) <= 1e-13 + 8 * eps / np.maximum(sf[:kf], 1e-300))
+    P0, P1 = Uf @ Vf, Ue @ Ve
+    skeep = sf[kf - 1]
+    assert np.linalg.norm(P0 - P1) < (1e-12 + 100 * eps / skeep) * np.linalg.norm(P0)
+    assert np.abs(Ue.T @ Ue - np.eye(ke)).max() < 1e-12
+    # the reduction stopped early: at ~45% of the order on the large cases,
+    # which saves 1 - (1 - 0.45)^3 ~ 16% of its bytes (the first columns
+    # stream the largest trailing blocks) and ~40% of its time (two
+    # launches per column, latency bound); the first check comes at column
+    # 256, so the small case saves only its last columns
+    assert be < (0.9 if m >= 1000 else 1.0) * bf
+
+
+def test_sweeps_early_stop_match_full_reduction(engine):
+    """Two D=256 Hubbard sweeps (N=16, Gram orders up to 1024) from the same
+    state with the early-stopping reduction (mode 1) and the complete one
+    (mode 3): identical keep counts at every bond and energies to 1e-12 per
+    site."""
+    from dmrgpy_amd import algorithms as alg
+    from dmrgpy_amd.chain import Chain
+    from dmrgpy_amd.multioperator import hubbard_terms
+    n = 16
+    ch = Chain([1] * n, field="real")
+    ch.set_hamiltonian(hubbard_terms(n, 1.0, 4.0))
+    engine = ch.eng
+    np.random.seed(5)
+    psi = ch._default_mps(maxm=64)
+    for m in (64, 128):
+        alg.dmrg_sweep(engine, psi, ch.H, m, 1e-12, 2, 0)
+    res = {}
+    try:
+        for mode in (3, 1):
+            engine.set_eigensolver(mode)
+            p = psi.copy()
+            st = alg.Stats()
+            engine.prof_enable(True)
+            es = [alg.dmrg_sweep(engine, p, ch.H, 256, 1e-12, 2, 0, st) for _ in range(2)]
+            nbytes = engine.prof_read_bytes()[4]
+            engine.prof_enable(False)
+            res[mode] = (es, [t[1] for t in st.truncation], [t[2] for t in st.truncation], nbytes)
+    finally:
+        engine.set_eigensolver(1)
+    (ef, kf, df, bf), (ee, ke, de, be) = res[3], res[1]
+    print("energies full", ef, "early", ee, "max keep", max(kf), "reduction bytes ratio", be / bf, flush=True)
+    assert be <= bf
+    assert kf == ke
+    assert max(abs(a - b) for a, b in zip(ef, ee)) / n < 1e-12
+    assert max(abs(a - b) for a, b in zip(df, de)) < 1e-14
