@@ -207,6 +207,8 @@ int dmrg_ctx_destroy(dmrg_ctx* ctx) {
     (void)hipEventDestroy(p.stop);
   }
   for (auto e : c.event_pool) (void)hipEventDestroy(e);
+  for (auto e : c.trd_ev)
+    if (e) (void)hipEventDestroy(e);
   for (int i = 0; i < dmrg::WS_NSLOTS; ++i) c.ws[i].release();
   if (c.host_scalars) (void)hipHostFree(c.host_scalars);
   if (c.blas) rocblas_destroy_handle(c.blas);

@@ -110,6 +110,7 @@ struct Ctx {
   // with rocSOLVER ormtr / unmtr for the back-transformation; 0 = rocSOLVER
   // syevd / heevd
   int eig_mode = 1;
+  hipEvent_t trd_ev[2] = {nullptr, nullptr};   // early-stop checks of the reduction in flight
   // allocation-free calls (dmrg_heff_apply_noalloc): while `noalloc` is set,
   // ws_get carves the workspace out of this device's static scratch array
   // (device address `scratch`, mapped at context creation) instead of the

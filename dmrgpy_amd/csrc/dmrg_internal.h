@@ -68,15 +68,17 @@ bool smallop_sparse_ok(const SmallOp& s);
 int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol = 0.0);
 // zhetrd (uplo 'L') of the column-major view of a Hermitian n x n planar
 // complex buffer (plane stride n*n) in place: d[n], e[n-1] real, tau[n-1]
-// complex interleaved
-void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau);
+// complex interleaved; returns the number of reflectors computed (as
+// sytrd_lower, with the same early stop over both planes)
+int hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol = 0.0);
 // rows of Zt (keep x n, row-major: eigenvectors of the tridiagonal) <- rows
 // of Q Y for the first nrefl reflectors sytrd_lower left in A (all n - 1 by
 // default; dormtr L/lower/N by blocked compact-WY GEMMs)
 void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Zt, int keep, int nrefl = -1);
 // complex: rows of Et (keep x n, planar, plane stride keep*n) <- rows of Q Y
-// for hetrd_lower's reflectors (zunmtr L/lower/N)
-void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Et, int keep);
+// for the first nrefl of hetrd_lower's reflectors (all n - 1 by default;
+// zunmtr L/lower/N)
+void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Et, int keep, int nrefl = -1);
 GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag);
 
 double heff_flops(const dmrg_heff_desc& d);

@@ -462,12 +462,20 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
     d.pA = mn; d.pB = mn; d.pC = (long long)g * g;
     gemm_f64(c, d, Ms, Ms, Gplanar);
     if (cx && c->eig_mode >= 1) {
-      // complex Hermitian: own reduction to a real tridiagonal, rocSOLVER's
-      // divide and conquer on it; unmtr on the kept columns after keep
+      // complex Hermitian: own reduction to a real tridiagonal (stopping
+      // early in mode 1 as the real one does), rocSOLVER's divide and
+      // conquer on it; unmtr on the kept columns after keep
       trd_Z = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * ((size_t)g * g + 2 * (size_t)g + 16));
       trd_tau = trd_Z + (size_t)g * g;
-      hetrd_lower(c, Gplanar, g, ev, ework, trd_tau);
-      check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, g, ev, ework, trd_Z, g, info), "dstedc");
+      trd_nrefl = hetrd_lower(c, Gplanar, g, ev, ework, trd_tau, c->eig_mode == 1 ? TRD_STOP_TOL : 0.0);
+      trd_m = trd_nrefl < g - 1 ? trd_nrefl + 1 : g;
+      if (trd_m < g) {
+        trd_Zs = (double*)ws_get(c, WS_TRD_ZS, sizeof(double) * (size_t)trd_m * trd_m);
+        check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, trd_m, ev, ework, trd_Zs, trd_m, info),
+                 "dstedc");
+      } else {
+        check_rb(rocsolver_dstedc(c->blas, rocblas_evect_tridiagonal, g, ev, ework, trd_Z, g, info), "dstedc");
+      }
       trd_refl = Gplanar;
     } else if (cx) {
       double* Gi = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)g * g);
@@ -548,10 +556,15 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
     const long long kg = (long long)keep * g;
     if (c->eig_mode != 2) {   // own blocked product on planar rows
       double* Et = (double*)ws_get(c, WS_SVD_CPLX, sizeof(double) * 2 * (size_t)kg);
-      DMRG_CHECK_HIP(hipMemcpyAsync(Et, trd_Z + (size_t)(g - keep) * g, sizeof(double) * kg,
-                                    hipMemcpyDeviceToDevice, c->stream));
+      if (trd_m < g) {   // the kept eigenvectors of the leading block, zero beyond it
+        embed_rows_k<<<g1(kg), 256, 0, c->stream>>>(trd_Zs + (size_t)(trd_m - keep) * trd_m, trd_m, keep, g, Et);
+        DMRG_CHECK_HIP(hipGetLastError());
+      } else {
+        DMRG_CHECK_HIP(hipMemcpyAsync(Et, trd_Z + (size_t)(g - keep) * g, sizeof(double) * kg,
+                                      hipMemcpyDeviceToDevice, c->stream));
+      }
       DMRG_CHECK_HIP(hipMemsetAsync(Et + kg, 0, sizeof(double) * kg, c->stream));
-      unmtr_lower_rows(c, trd_refl, g, trd_tau, Et, keep);
+      unmtr_lower_rows(c, trd_refl, g, trd_tau, Et, keep, trd_nrefl);
       planar_to_inter_k<<<g1(kg), 256, 0, c->stream>>>(Et, kg, Zk);
       DMRG_CHECK_HIP(hipGetLastError());
     } else {   // rocSOLVER zunmtr on interleaved reflectors

@@ -335,17 +335,18 @@ __global__ __launch_bounds__(256) void trd_fin_k(int n, int s, int k, const doub
 // Sum of squares of the trailing block A[s:, s:] (pitch n): per-block
 // partial sums over whole rows, then one fixed-order sum (deterministic).
 constexpr int TRD_SQ_BLOCKS = 512;
-__global__ __launch_bounds__(256) void trd_sumsq_k(const double* __restrict__ A, int n, int s,
+__global__ __launch_bounds__(256) void trd_sumsq_k(const double* __restrict__ A, int n, int s, int planes,
                                                    double* __restrict__ part) {
   __shared__ double red4[4];
   double acc = 0.0;
-  for (int r = s + blockIdx.x; r < n; r += gridDim.x) {
-    const double* row = A + (size_t)r * n;
-    for (int cc = s + threadIdx.x; cc < n; cc += 256) {
-      const double x = row[cc];
-      acc += x * x;
+  for (int pl = 0; pl < planes; ++pl)   // planar complex: both planes (stride n*n)
+    for (int r = s + blockIdx.x; r < n; r += gridDim.x) {
+      const double* row = A + (size_t)pl * n * n + (size_t)r * n;
+      for (int cc = s + threadIdx.x; cc < n; cc += 256) {
+        const double x = row[cc];
+        acc += x * x;
+      }
     }
-  }
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = acc;
   __syncthreads();
@@ -359,28 +360,39 @@ __global__ __launch_bounds__(256) void trd_sumpart_k(const double* __restrict__ 
   if (threadIdx.x == 0) *out = v;
 }
 
-// d[s..n-1] = 0, e[s..n-2] = 0, tau[s..n-2] = 0: the tridiagonal of a
-// reduction stopped after s columns (the trailing block taken as zero)
+// d[s..n-1] = 0, e[s..n-2] = 0, tau[s..n-2] = 0 (tstride doubles per tau:
+// 2 for interleaved complex): the tridiagonal of a reduction stopped after s
+// columns (the trailing block taken as zero)
 __global__ void trd_clear_tail_k(int n, int s, double* __restrict__ d, double* __restrict__ e,
-                                 double* __restrict__ tau) {
+                                 double* __restrict__ tau, int tstride) {
   for (int i = s + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     d[i] = 0.0;
     if (i < n - 1) {
       e[i] = 0.0;
-      tau[i] = 0.0;
+      for (int q = 0; q < tstride; ++q) tau[(size_t)i * tstride + q] = 0.0;
     }
   }
 }
 
-static double trailing_sumsq(Ctx* c, const double* A, int n, int s, double* part, int cap, double* out) {
+// |A[s:, s:]|_F^2 into host_scalars[32 + slot] (pinned), with event
+// trd_ev[slot] recorded after the copy: the caller waits on the event, so a
+// check can stay in flight while the next panel is queued
+static void trailing_sumsq_async(Ctx* c, const double* A, int n, int s, int planes, double* part, int cap,
+                                 double* out, int slot) {
   const int nb = std::max(1, std::min(std::min(TRD_SQ_BLOCKS, n - s), cap));
-  trd_sumsq_k<<<nb, 256, 0, c->stream>>>(A, n, s, part);
+  trd_sumsq_k<<<nb, 256, 0, c->stream>>>(A, n, s, planes, part);
   DMRG_CHECK_HIP(hipGetLastError());
-  trd_sumpart_k<<<1, 256, 0, c->stream>>>(part, nb, out);
+  trd_sumpart_k<<<1, 256, 0, c->stream>>>(part, nb, out + slot);
   DMRG_CHECK_HIP(hipGetLastError());
-  DMRG_CHECK_HIP(hipMemcpyAsync(c->host_scalars + 32, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
-  return c->host_scalars[32];
+  DMRG_CHECK_HIP(hipMemcpyAsync(c->host_scalars + 32 + slot, out + slot, sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
+  if (!c->trd_ev[slot]) DMRG_CHECK_HIP(hipEventCreateWithFlags(&c->trd_ev[slot], hipEventDisableTiming));
+  DMRG_CHECK_HIP(hipEventRecord(c->trd_ev[slot], c->stream));
+}
+
+static double trailing_sumsq_wait(Ctx* c, int slot) {
+  DMRG_CHECK_HIP(hipEventSynchronize(c->trd_ev[slot]));
+  return c->host_scalars[32 + slot];
 }
 
 // A (n x n, symmetric, row-major = column-major) -> d[n], e[n-1], tau[n-1]
@@ -389,7 +401,12 @@ static double trailing_sumsq(Ctx* c, const double* A, int n, int s, double* part
 // each panel (from column TRD_STOP_MIN on) the Frobenius norm of the
 // trailing block is compared with stop_tol |A|_F, and once it is below, the
 // block is taken as zero: the remaining d, e and tau are set to 0 and the
-// tridiagonal is complete after s columns.  That perturbs A by at most
+// tridiagonal is complete after s columns.  The host reads a panel's check
+// while the next panel is already queued (no idle GPU between panels), so
+// the reduction stops one panel after the first panel whose trailing block
+// passed; the trailing norm never grows (each step is an orthogonal
+// similarity of the trailing block, of which the next one is a principal
+// submatrix), so the later block passes too.  That perturbs A by at most
 // stop_tol |A|_F (every eigenvalue by at most as much), far below the
 // rounding of the reduction itself at stop_tol ~ 2e-16; the dense
 // eigenproblem then shrinks to order s + 1 plus zeros.
@@ -414,11 +431,15 @@ int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, dou
   // early stop: |A|_F of the input, and scratch for the trailing sums (the
   // launch-1 partial buffer is free between panels)
   const bool early = stop_tol > 0.0 && n > TRD_STOP_MIN + TRD_NB;
-  double* sq_out = part2 + gmax2;    // 8 spare doubles before the state
+  double* sq_out = part2 + gmax2;    // 8 spare doubles before the state (two used)
   double stop2 = 0.0;
   const int sq_cap = gmax1 * TRD_PS;   // partial sums the launch-1 buffer holds
-  if (early) stop2 = stop_tol * stop_tol * trailing_sumsq(c, A, n, 0, part1, sq_cap, sq_out);
+  if (early) {
+    trailing_sumsq_async(c, A, n, 0, 1, part1, sq_cap, sq_out, 0);
+    stop2 = stop_tol * stop_tol * trailing_sumsq_wait(c, 0);
+  }
   int done = n - 1;   // reflectors computed
+  int pend = -1;      // slot of a trailing-norm check in flight
   int g2prev = 1;
   for (int j0 = 0; j0 < n - 1; j0 += TRD_NB) {
     const int nbp = std::min(TRD_NB, n - 1 - j0);
@@ -454,13 +475,20 @@ int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, dou
         gemm_f64(c, g, Vs, Vs + TRD_NB, C);
         gemm_f64(c, g, Vs + TRD_NB, Vs, C);
       }
-      if (early && nbp == TRD_NB && s >= TRD_STOP_MIN && s < n - 1 &&
-          trailing_sumsq(c, A, n, s, part1, sq_cap, sq_out) <= stop2) {
-        trd_clear_tail_k<<<(n - s + 255) / 256, 256, 0, c->stream>>>(n, s, d, e, tau);
+      int slot = -1;
+      if (early && nbp == TRD_NB && s >= TRD_STOP_MIN && s < n - 1) {
+        slot = pend == 0 ? 1 : 0;
+        trailing_sumsq_async(c, A, n, s, 1, part1, sq_cap, sq_out, slot);
+      }
+      // the previous panel's check: its block passed => this one (already
+      // queued and applied) passes too; stop here
+      if (pend >= 0 && trailing_sumsq_wait(c, pend) <= stop2 && s < n - 1) {
+        trd_clear_tail_k<<<(n - s + 255) / 256, 256, 0, c->stream>>>(n, s, d, e, tau, 1);
         DMRG_CHECK_HIP(hipGetLastError());
         done = s;
         break;
       }
+      pend = slot;
     }
   }
   if (done == n - 1)
@@ -857,8 +885,10 @@ __global__ __launch_bounds__(256) void trc_fin_k(int n, int s, int k, const doub
 
 // A (n x n, planar complex with plane stride n*n, Hermitian; the buffer read
 // column-major) -> d[n], e[n-1] (real), tau[n-1] (complex, interleaved) and
-// reflectors in place (zhetrd, uplo 'L', on the column-major view).
-void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
+// reflectors in place (zhetrd, uplo 'L', on the column-major view).  Returns
+// the number of reflectors computed; stop_tol > 0 stops the reduction as
+// sytrd_lower does (trailing block, both planes, below stop_tol |A|_F).
+int hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol) {
   DMRG_REQUIRE(n >= 2, "hetrd: order must be at least 2");
   const size_t vw = (size_t)n * TRD_P;   // one plane of the [W | V | W] panel store
   const int gmax1 = (n + TRD_ROWS1 - 1) / TRD_ROWS1;
@@ -874,10 +904,17 @@ void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
   double* part2 = part1 + (size_t)gmax1 * TRC_PS;
   TrcState* st = reinterpret_cast<TrcState*>(part2 + 2 * (size_t)gmax2 + 8);
   DMRG_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(TrcState), c->stream));
-  // algorithmic bytes: the lower triangle of the trailing matrix (both planes) once per column
-  double bytes = 0.0;
-  for (int j = 0; j < n - 1; ++j) bytes += 16.0 * (double)(n - j - 1) * (n - j) / 2.0;
-  ProfScope ps(c, 4, 0.0, bytes);
+  ProfScope ps(c, 4, 0.0, 0.0);
+  const bool early = stop_tol > 0.0 && n > TRD_STOP_MIN + TRD_NB;
+  double* sq_out = part2 + 2 * (size_t)gmax2;   // 8 spare doubles before the state (two used)
+  double stop2 = 0.0;
+  const int sq_cap = gmax1 * TRC_PS;
+  if (early) {
+    trailing_sumsq_async(c, A, n, 0, 2, part1, sq_cap, sq_out, 0);
+    stop2 = stop_tol * stop_tol * trailing_sumsq_wait(c, 0);
+  }
+  int done = n - 1;
+  int pend = -1;
   int g2prev = 1;
   for (int j0 = 0; j0 < n - 1; j0 += TRD_NB) {
     const int nbp = std::min(TRD_NB, n - 1 - j0);
@@ -912,10 +949,26 @@ void hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau) {
       const double* Ws = W + (size_t)s * TRD_P;
       gemm_f64(c, g, Ws, Ws + TRD_NB, C);
       if (!full) gemm_f64(c, g, Ws + TRD_NB, Ws, C);   // short last panel: columns past nbp are stale
+      int slot = -1;
+      if (early && full && s >= TRD_STOP_MIN && s < n - 1) {
+        slot = pend == 0 ? 1 : 0;
+        trailing_sumsq_async(c, A, n, s, 2, part1, sq_cap, sq_out, slot);
+      }
+      if (pend >= 0 && trailing_sumsq_wait(c, pend) <= stop2 && s < n - 1) {
+        trd_clear_tail_k<<<(n - s + 255) / 256, 256, 0, c->stream>>>(n, s, d, e, tau, 2);
+        DMRG_CHECK_HIP(hipGetLastError());
+        done = s;
+        break;
+      }
+      pend = slot;
     }
   }
-  DMRG_CHECK_HIP(hipMemcpyAsync(d + (n - 1), A + (size_t)(n - 1) * n + (n - 1), sizeof(double),
-                                hipMemcpyDeviceToDevice, c->stream));
+  if (done == n - 1)
+    DMRG_CHECK_HIP(hipMemcpyAsync(d + (n - 1), A + (size_t)(n - 1) * n + (n - 1), sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
+  // algorithmic bytes: the lower triangle of the trailing matrix (both planes) once per reduced column
+  for (int j = 0; j < done; ++j) ps.bytes += 16.0 * (double)(n - j - 1) * (n - j) / 2.0;
+  return done;
 }
 
 
@@ -1088,12 +1141,13 @@ void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
 constexpr int BTC_NB = 64;
 constexpr int BTC_TRI = BTC_NB * (BTC_NB + 1) / 2;
 
-__global__ void btc_vall_k(const double* __restrict__ A, int n, long long total, double* __restrict__ Vall) {
+__global__ void btc_vall_k(const double* __restrict__ A, int n, int nrefl, long long total,
+                           double* __restrict__ Vall) {
   const size_t pA = (size_t)n * n;
   for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long long)gridDim.x * 256) {
     const int j = (int)(idx / n), r = (int)(idx % n);
     double vr = 0.0, vi = 0.0;
-    if (j < n - 1) {
+    if (j < nrefl) {
       if (r == j + 1) {
         vr = 1.0;
       } else if (r > j + 1) {
@@ -1111,15 +1165,15 @@ __device__ __forceinline__ int btc_t_idx(int k, int l) { return k * BTC_NB - k *
 
 // S planar (plane stride pS), tau interleaved; TtS planar (plane stride pT)
 __global__ __launch_bounds__(BTC_NB) void btc_larft_k(const double* __restrict__ S, long long pS,
-                                                     const double* __restrict__ tau, int n,
+                                                     const double* __restrict__ tau, int nrefl,
                                                      double* __restrict__ TtS, long long pT) {
   __shared__ double sSr[BTC_TRI], sSi[BTC_TRI], sTr[BTC_TRI], sTi[BTC_TRI];
   __shared__ double str[BTC_NB], sti[BTC_NB];
   const int b = blockIdx.x, k = threadIdx.x;
   const int j0 = b * BTC_NB;
   const double* Sb = S + (size_t)b * BTC_NB * BTC_NB;
-  str[k] = j0 + k < n - 1 ? tau[2 * (j0 + k)] : 0.0;
-  sti[k] = j0 + k < n - 1 ? tau[2 * (j0 + k) + 1] : 0.0;
+  str[k] = j0 + k < nrefl ? tau[2 * (j0 + k)] : 0.0;
+  sti[k] = j0 + k < nrefl ? tau[2 * (j0 + k) + 1] : 0.0;
 #pragma unroll 16
   for (int t = 0; t < BTC_NB; ++t) {
     const double vr = Sb[(size_t)t * BTC_NB + k], vi = Sb[(size_t)t * BTC_NB + k + pS];
@@ -1164,11 +1218,13 @@ __global__ __launch_bounds__(BTC_NB) void btc_larft_k(const double* __restrict__
 }
 
 // Et (keep x n planar, plane stride keep*n) <- rows of Q Y for hetrd_lower's
-// reflectors in A (planar, plane stride n*n) and tau (interleaved)
-void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Et, int keep) {
-  DMRG_REQUIRE(n >= 2 && keep >= 1 && keep <= n, "unmtr: bad sizes");
+// reflectors in A (planar, plane stride n*n) and tau (interleaved); nrefl
+// reflectors (n - 1 when negative)
+void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Et, int keep, int nrefl) {
+  if (nrefl < 0) nrefl = n - 1;
+  DMRG_REQUIRE(n >= 2 && keep >= 1 && keep <= n && nrefl >= 1 && nrefl <= n - 1, "unmtr: bad sizes");
   const int NB = BTC_NB;
-  const int nblk = (n - 1 + NB - 1) / NB;
+  const int nblk = (nrefl + NB - 1) / NB;
   const size_t vrows = (size_t)nblk * NB;
   const long long pV = (long long)vrows * n;
   double* Vall = (double*)ws_get(c, WS_BT_V, sizeof(double) * 2 * (size_t)pV);
@@ -1178,7 +1234,7 @@ void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
   double* TtS = Sm + 2 * nS;
   double* Ycat = TtS + 2 * nT;
   double* Y2 = Ycat + 2 * nY;
-  btc_vall_k<<<(int)std::min<long long>((pV + 255) / 256, 65536), 256, 0, c->stream>>>(A, n, pV, Vall);
+  btc_vall_k<<<(int)std::min<long long>((pV + 255) / 256, 65536), 256, 0, c->stream>>>(A, n, nrefl, pV, Vall);
   DMRG_CHECK_HIP(hipGetLastError());
   {  // S_b = V_b^H V_b (rows of Vall are the reflectors): conj(Vall_b) Vall_b^T
     GemmDesc g = gdesc(NB, NB, n, n, n, NB, CPLX, 4);
@@ -1191,7 +1247,7 @@ void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
     g.pC = (long long)nS;
     gemm_f64(c, g, Vall, Vall, Sm);
   }
-  btc_larft_k<<<nblk, NB, 0, c->stream>>>(Sm, (long long)nS, tau, n, TtS, (long long)nT);
+  btc_larft_k<<<nblk, NB, 0, c->stream>>>(Sm, (long long)nS, tau, nrefl, TtS, (long long)nT);
   DMRG_CHECK_HIP(hipGetLastError());
   const long long pE = (long long)keep * n;
   for (int b = nblk - 1; b >= 0; --b) {

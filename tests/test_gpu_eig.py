@@ -405,8 +405,9 @@ def test_large_hermitian_order(engine):
 # ---- early stop of the reduction (mode 1) against the complete reduction
 # (mode 3): the split must keep the same count and discard the same weight
 
-@pytest.mark.parametrize("shape", [(2400, 2560), (1800, 1700), (330, 340)])
-def test_split_early_stop_matches_full_reduction(engine, shape):
+@pytest.mark.parametrize("field,shape", [(0, (2400, 2560)), (0, (1800, 1700)), (0, (330, 340)),
+                                         (1, (1600, 1700)), (1, (1400, 1300))])
+def test_split_early_stop_matches_full_reduction(engine, field, shape):
     """A graded two-site-like matrix M = diag(r) Z diag(c) (Z Gaussian; r, c
     falling by 1e-10 across the rows / columns, as Schmidt weights of the
     neighbouring bonds grade a DMRG theta), so the Gram matrix's rounding
@@ -416,29 +417,38 @@ def test_split_early_stop_matches_full_reduction(engine, shape):
     then agree with the complete reduction (mode 3) -- keep count,
     discarded weight to 1e-14, kept singular values and the truncated
     product within the Gram-route rounding bound, isometry -- and must have
-    reduced fewer columns (algorithmic bytes of the tridiagonalisation)."""
+    reduced fewer columns (algorithmic bytes of the tridiagonalisation).
+    Field 1: the same with a complex Gaussian Z (Hermitian reduction)."""
     import time
     import torch
     rng = np.random.default_rng(shape[0])
     m, n = shape
     r = 10.0 ** (-10.0 * np.arange(m) / m)
     c = 10.0 ** (-10.0 * np.arange(n) / n)
-    M = (r[:, None] * rng.standard_normal((m, n))) * c[None, :]
-    Md = _dev(engine, M[None])
+    Z = rng.standard_normal((m, n))
+    if field:
+        Z = Z + 1j * rng.standard_normal((m, n))
+    M = (r[:, None] * Z) * c[None, :]
+    Md = _dev(engine, np.stack([M.real, M.imag]) if field else M[None])
     out = {}
     try:
         for mode in (3, 1):
             engine.set_eigensolver(mode)
-            engine.svd_split(0, Md, m, n, 1e-12, 0, 1, 0)     # warm-up (workspace)
+            engine.svd_split(field, Md, m, n, 1e-12, 0, 1, 0)     # warm-up (workspace)
             engine.prof_enable(True)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            U, V, keep, disc, sv = engine.svd_split(0, Md, m, n, 1e-12, 0, 1, 0)
+            U, V, keep, disc, sv = engine.svd_split(field, Md, m, n, 1e-12, 0, 1, 0)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             trd_bytes = engine.prof_read_bytes()[4]
             engine.prof_enable(False)
-            out[mode] = (U.cpu().numpy()[0], V.cpu().numpy()[0], keep, disc, sv, dt, trd_bytes)
+            Uh, Vh = U.cpu().numpy(), V.cpu().numpy()
+            if field:
+                Uh, Vh = Uh[0] + 1j * Uh[1], Vh[0] + 1j * Vh[1]
+            else:
+                Uh, Vh = Uh[0], Vh[0]
+            out[mode] = (Uh, Vh, keep, disc, sv, dt, trd_bytes)
     finally:
         engine.set_eigensolver(1)   # the default
     Uf, Vf, kf, df, sf, tf, bf = out[3]
@@ -452,7 +462,7 @@ def test_split_early_stop_matches_full_reduction(engine, shape):
     P0, P1 = Uf @ Vf, Ue @ Ve
     skeep = sf[kf - 1]
     assert np.linalg.norm(P0 - P1) < (1e-12 + 100 * eps / skeep) * np.linalg.norm(P0)
-    assert np.abs(Ue.T @ Ue - np.eye(ke)).max() < 1e-12
+    assert np.abs(Ue.conj().T @ Ue - np.eye(ke)).max() < 1e-12
     # the reduction stopped early: at ~45% of the order on the large cases,
     # which saves 1 - (1 - 0.45)^3 ~ 16% of its bytes (the first columns
     # stream the largest trailing blocks) and ~40% of its time (two
@@ -461,16 +471,18 @@ def test_split_early_stop_matches_full_reduction(engine, shape):
     assert be < (0.9 if m >= 1000 else 1.0) * bf
 
 
-def test_sweeps_early_stop_match_full_reduction(engine):
+@pytest.mark.parametrize("field", ["real", "auto"])
+def test_sweeps_early_stop_match_full_reduction(engine, field):
     """Two D=256 Hubbard sweeps (N=16, Gram orders up to 1024) from the same
     state with the early-stopping reduction (mode 1) and the complete one
     (mode 3): identical keep counts at every bond and energies to 1e-12 per
-    site."""
+    site; real tensors, and complex ones (the reference's random complex
+    start, field "auto") through the Hermitian reduction."""
     from dmrgpy_amd import algorithms as alg
     from dmrgpy_amd.chain import Chain
     from dmrgpy_amd.multioperator import hubbard_terms
     n = 16
-    ch = Chain([1] * n, field="real")
+    ch = Chain([1] * n, field=field)
     ch.set_hamiltonian(hubbard_terms(n, 1.0, 4.0))
     engine = ch.eng
     np.random.seed(5)
