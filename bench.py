@@ -279,6 +279,7 @@ def workload_tdvp(args):
     eng = ch.eng
     np.random.seed(0)
     psi = ch._default_mps(maxm=64)
+    guided = args.svd == "guided"
     t0 = time.perf_counter()
     for m in [m for m in (64, 128, 256, 512) if m < D] + [D, D]:
         e = alg.dmrg_sweep(eng, psi, ch.H, m, 1e-12, 2, 1, None, res_tol=RES_TOL[1])
@@ -294,6 +295,7 @@ def workload_tdvp(args):
     norm0 = float(abs(alg.overlap(eng, psi1, psi1))) ** 0.5
     dt = 0.05
     stats = alg.Stats()
+    eng.svd_guided = guided   # the TDVP splits only (the ground state above used the full split)
 
     def step():
         alg.tdvp_step(eng, psi1, Hs, dt, 1e-12, D, 20, 0, stats)
@@ -312,8 +314,9 @@ def workload_tdvp(args):
         corr.append(alg.overlap(eng, psi2, psi1))
         step()
         if (k + 1) % 10 == 0:
-            log("tdvp step {} t={:.2f} max bond {} elapsed {:.1f}s C={}".format(
-                k + 1, (k + 1) * dt, max(psi1.bond_dims()), time.perf_counter() - ts, corr[-1]))
+            log("tdvp step {} t={:.2f} max bond {} elapsed {:.1f}s C={}{}".format(
+                k + 1, (k + 1) * dt, max(psi1.bond_dims()), time.perf_counter() - ts, corr[-1],
+                " guided {:.2f}".format(sum(stats.svd_guided) / max(1, len(stats.svd_guided))) if guided else ""))
     torch.cuda.synchronize()
     dt_s = (time.perf_counter() - ts) / args.steps
     tms, flops, calls = eng.prof_read()
@@ -323,8 +326,10 @@ def workload_tdvp(args):
            "ms_per_step": dt_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "fp64", "data": "synthetic: DMRG ground state of the model, seeded random initial MPS",
            "config": {"workload": "TDVP quench, open S=1 Heisenberg chain N={}, d=3, w=5, D cap {}, dt=0.05, "
-                                  "Krylov dim 20, Sz at site {}".format(n, D, n // 2),
-                      "N": n, "D": D, "E0": e0},
+                                  "Krylov dim 20, Sz at site {}{}".format(
+                                      n, D, n // 2, ", subspace-guided splits" if guided else ""),
+                      "N": n, "D": D, "E0": e0, "svd": args.svd},
+           "svd_guided_fraction": sum(stats.svd_guided) / max(1, len(stats.svd_guided)),
            "max_bond": max(psi1.bond_dims()),
            "krylov_iters_per_call": sum(stats.krylov_iters) / max(1, len(stats.krylov_iters)),
            "time_split_s_per_step": {"heff": tms[0] / 1e3 / args.steps, "env": tms[1] / 1e3 / args.steps,
