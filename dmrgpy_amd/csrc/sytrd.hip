@@ -1002,52 +1002,107 @@ __global__ void bt_vall_k(const double* __restrict__ A, int n, int nrefl, long l
   }
 }
 
-// T_b from S_b = V_b V_b^T and tau (one workgroup per block, thread k owns
-// row k of T: T[k][k] = tau_k, T[k][i] = -tau_i sum_{l=k}^{i-1} T[k][l] S[l][i]),
-// then BT_SPLIT stacked copies of T_b^T.  The strict upper triangle of S
-// (column-packed) and the upper triangle of T (row-packed) sit in LDS, so
-// the recurrence runs without global loads or barriers.
+// T_b from S_b = V_b V_b^T and tau (one workgroup of 256 threads per block),
+// then BT_SPLIT stacked copies of T_b^T.  T is built by blocked recursion:
+// its 16 x 16 diagonal blocks by the column recurrence (thread k owns row
+// k: T[k][k] = tau_k, T[k][i] = -tau_i sum_{l=k}^{i-1} T[k][l] S[l][i]),
+// then pairs of diagonal blocks of size h = 16, 32, 64 are joined by the
+// compact-WY identity T12 = -T1 (S12 T2) (LAPACK dlarft's recursive form:
+// (I - V1 T1 V1^T)(I - V2 T2 V2^T) = I - V T V^T with that T12), every
+// output an independent dot product.  The serial recurrence over the whole
+// block made row 0 a chain of 127 dependent steps (279 us per call).  The
+// strict upper triangle of S (column-packed) and the upper triangle of T
+// (row-packed) sit in LDS; X = S12 T2 is kept in T12's slots until
+// T12 = -T1 X replaces it (through registers, so no slot is read after it
+// is overwritten).
 constexpr int BT_TRI = BT_NB * (BT_NB + 1) / 2;
 __device__ __forceinline__ int bt_s_idx(int l, int i) { return i * (i - 1) / 2 + l; }            // l < i
 __device__ __forceinline__ int bt_t_idx(int k, int l) { return k * BT_NB - k * (k - 1) / 2 + (l - k); }  // l >= k
+constexpr int BT_LEAF = 16;
 
-__global__ __launch_bounds__(BT_NB) void bt_larft_k(const double* __restrict__ S, const double* __restrict__ tau,
-                                                   int nrefl, double* __restrict__ TtS) {
-  __shared__ double sS[BT_TRI];
+__global__ __launch_bounds__(256) void bt_larft_k(const double* __restrict__ S, const double* __restrict__ tau,
+                                                  int nrefl, double* __restrict__ TtS) {
+  __shared__ double sS[BT_NB * (BT_NB - 1) / 2];
   __shared__ double sT[BT_TRI];
   __shared__ double stau[BT_NB];
-  const int b = blockIdx.x, k = threadIdx.x;
+  const int b = blockIdx.x, tid = threadIdx.x;
   const int j0 = b * BT_NB;
   const double* Sb = S + (size_t)b * BT_NB * BT_NB;
-  stau[k] = j0 + k < nrefl ? tau[j0 + k] : 0.0;
-#pragma unroll 16
-  for (int t = 0; t < BT_NB; ++t) {   // S[t][k], kept for t < k (column k of the strict upper triangle)
-    const double v = Sb[(size_t)t * BT_NB + k];
-    if (t < k) sS[bt_s_idx(t, k)] = v;
+  if (tid < BT_NB) stau[tid] = j0 + tid < nrefl ? tau[j0 + tid] : 0.0;
+  for (int idx = tid; idx < BT_NB * BT_NB; idx += 256) {   // S[t][k], kept for t < k
+    const int t = idx / BT_NB, k = idx % BT_NB;
+    if (t < k) sS[bt_s_idx(t, k)] = Sb[idx];
   }
   __syncthreads();
-  sT[bt_t_idx(k, k)] = stau[k];
-  for (int i = k + 1; i < BT_NB; ++i) {
-    const double ti = stau[i];
-    // four independent partial sums (fixed order): the chain of dependent
-    // additions, not the LDS reads, bounds this recurrence
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    const int tk = bt_t_idx(k, 0), si = bt_s_idx(0, i);   // row k of T and column i of S are contiguous
-    int l = k;
-    for (; l + 3 < i; l += 4) {
-      a0 += sT[tk + l] * sS[si + l];
-      a1 += sT[tk + l + 1] * sS[si + l + 1];
-      a2 += sT[tk + l + 2] * sS[si + l + 2];
-      a3 += sT[tk + l + 3] * sS[si + l + 3];
+  if (tid < BT_NB) {   // diagonal blocks of BT_LEAF by the recurrence
+    const int k = tid, end = (k / BT_LEAF + 1) * BT_LEAF;
+    const int tk = bt_t_idx(k, 0);   // row k of T is contiguous
+    sT[tk + k] = stau[k];
+    for (int i = k + 1; i < end; ++i) {
+      const int si = bt_s_idx(0, i);   // column i of S is contiguous
+      double a = 0.0;
+      for (int l = k; l < i; ++l) a += sT[tk + l] * sS[si + l];
+      sT[tk + i] = -stau[i] * a;
     }
-    for (; l < i; ++l) a0 += sT[tk + l] * sS[si + l];
-    sT[bt_t_idx(k, i)] = -ti * ((a0 + a1) + (a2 + a3));
   }
   __syncthreads();
+  constexpr int MAXPER = BT_NB * (BT_NB / 2) / 2 / 256;   // outputs per thread at the last level (16)
+  for (int h = BT_LEAF; h < BT_NB; h *= 2) {
+    const int per = BT_NB * h / 2 / 256;   // (BT_NB / 2h) pairs x h x h outputs over 256 threads
+    double x[MAXPER];
+    // X = S12 T2 into the T12 slots (T2 is read, T12 only written: disjoint)
+#pragma unroll
+    for (int e = 0; e < MAXPER; ++e) {
+      if (e < per) {
+        const int o = tid + 256 * e;
+        const int p = o / (h * h), r = o % (h * h);
+        const int b0 = 2 * h * p, k = b0 + r / h, i = b0 + h + r % h;
+        double a = 0.0;
+        for (int l = b0 + h; l <= i; ++l) a += sS[bt_s_idx(k, l)] * sT[bt_t_idx(l, i)];
+        x[e] = a;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < MAXPER; ++e) {
+      if (e < per) {
+        const int o = tid + 256 * e;
+        const int p = o / (h * h), r = o % (h * h);
+        const int b0 = 2 * h * p, k = b0 + r / h, i = b0 + h + r % h;
+        sT[bt_t_idx(k, i)] = x[e];
+      }
+    }
+    __syncthreads();
+    // T12 = -T1 X
+#pragma unroll
+    for (int e = 0; e < MAXPER; ++e) {
+      if (e < per) {
+        const int o = tid + 256 * e;
+        const int p = o / (h * h), r = o % (h * h);
+        const int b0 = 2 * h * p, k = b0 + r / h, i = b0 + h + r % h;
+        const int tk = bt_t_idx(k, 0);
+        double a = 0.0;
+        for (int l = k; l < b0 + h; ++l) a += sT[tk + l] * sT[bt_t_idx(l, i)];
+        x[e] = -a;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < MAXPER; ++e) {
+      if (e < per) {
+        const int o = tid + 256 * e;
+        const int p = o / (h * h), r = o % (h * h);
+        const int b0 = 2 * h * p, k = b0 + r / h, i = b0 + h + r % h;
+        sT[bt_t_idx(k, i)] = x[e];
+      }
+    }
+    __syncthreads();
+  }
   double* out = TtS + (size_t)b * BT_SPLIT * BT_NB * BT_NB;
-  for (int s = 0; s < BT_SPLIT; ++s)
-    for (int i = 0; i < BT_NB; ++i)   // row k of T^T = column k of T
-      out[((size_t)s * BT_NB + k) * BT_NB + i] = i <= k ? sT[bt_t_idx(i, k)] : 0.0;
+  for (int idx = tid; idx < BT_NB * BT_NB; idx += 256) {   // row k of T^T = column k of T
+    const int k = idx / BT_NB, i = idx % BT_NB;
+    const double v = i <= k ? sT[bt_t_idx(i, k)] : 0.0;
+    for (int s = 0; s < BT_SPLIT; ++s) out[((size_t)s * BT_NB + k) * BT_NB + i] = v;
+  }
 }
 
 // Y[c][i] = sum_s Ycat[c][s nb + i] (the split inner products of a block, in
@@ -1093,7 +1148,7 @@ void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
     g.sC = (long long)BT_NB * BT_NB;
     gemm_f64(c, g, Vall, Vall, Sm);
   }
-  bt_larft_k<<<nblk, BT_NB, 0, c->stream>>>(Sm, tau, nrefl, TtS);
+  bt_larft_k<<<nblk, 256, 0, c->stream>>>(Sm, tau, nrefl, TtS);
   DMRG_CHECK_HIP(hipGetLastError());
   for (int b = nblk - 1; b >= 0; --b) {
     const int b0 = b * BT_NB;
@@ -1163,58 +1218,121 @@ __global__ void btc_vall_k(const double* __restrict__ A, int n, int nrefl, long 
 __device__ __forceinline__ int btc_s_idx(int l, int i) { return i * (i - 1) / 2 + l; }
 __device__ __forceinline__ int btc_t_idx(int k, int l) { return k * BTC_NB - k * (k - 1) / 2 + (l - k); }
 
-// S planar (plane stride pS), tau interleaved; TtS planar (plane stride pT)
-__global__ __launch_bounds__(BTC_NB) void btc_larft_k(const double* __restrict__ S, long long pS,
-                                                     const double* __restrict__ tau, int nrefl,
-                                                     double* __restrict__ TtS, long long pT) {
+// S planar (plane stride pS), tau interleaved; TtS planar (plane stride pT).
+// The same blocked recursion as bt_larft_k in complex arithmetic
+// (S[l][i] = v_l^H v_i, T12 = -T1 S12 T2 from (I - V1 T1 V1^H)(I - V2 T2 V2^H)),
+// 256 threads: 16 x 16 diagonal blocks by the recurrence, then levels 16, 32.
+__global__ __launch_bounds__(256) void btc_larft_k(const double* __restrict__ S, long long pS,
+                                                  const double* __restrict__ tau, int nrefl,
+                                                  double* __restrict__ TtS, long long pT) {
   __shared__ double sSr[BTC_TRI], sSi[BTC_TRI], sTr[BTC_TRI], sTi[BTC_TRI];
   __shared__ double str[BTC_NB], sti[BTC_NB];
-  const int b = blockIdx.x, k = threadIdx.x;
+  const int b = blockIdx.x, tid = threadIdx.x;
   const int j0 = b * BTC_NB;
   const double* Sb = S + (size_t)b * BTC_NB * BTC_NB;
-  str[k] = j0 + k < nrefl ? tau[2 * (j0 + k)] : 0.0;
-  sti[k] = j0 + k < nrefl ? tau[2 * (j0 + k) + 1] : 0.0;
-#pragma unroll 16
-  for (int t = 0; t < BTC_NB; ++t) {
-    const double vr = Sb[(size_t)t * BTC_NB + k], vi = Sb[(size_t)t * BTC_NB + k + pS];
+  if (tid < BTC_NB) {
+    str[tid] = j0 + tid < nrefl ? tau[2 * (j0 + tid)] : 0.0;
+    sti[tid] = j0 + tid < nrefl ? tau[2 * (j0 + tid) + 1] : 0.0;
+  }
+  for (int idx = tid; idx < BTC_NB * BTC_NB; idx += 256) {   // S[t][k], kept for t < k
+    const int t = idx / BTC_NB, k = idx % BTC_NB;
     if (t < k) {
-      sSr[btc_s_idx(t, k)] = vr;
-      sSi[btc_s_idx(t, k)] = vi;
+      sSr[btc_s_idx(t, k)] = Sb[idx];
+      sSi[btc_s_idx(t, k)] = Sb[idx + pS];
     }
   }
   __syncthreads();
-  sTr[btc_t_idx(k, k)] = str[k];
-  sTi[btc_t_idx(k, k)] = sti[k];
-  for (int i = k + 1; i < BTC_NB; ++i) {
-    double ar0 = 0.0, ai0 = 0.0, ar1 = 0.0, ai1 = 0.0;   // two independent partial sums (fixed order)
-    const int tk = btc_t_idx(k, 0), si = btc_s_idx(0, i);
-    int l = k;
-    for (; l + 1 < i; l += 2) {
-      const double xr = sTr[tk + l], xi = sTi[tk + l], yr = sSr[si + l], yi = sSi[si + l];
-      const double xr1 = sTr[tk + l + 1], xi1 = sTi[tk + l + 1], yr1 = sSr[si + l + 1], yi1 = sSi[si + l + 1];
-      ar0 += xr * yr - xi * yi;
-      ai0 += xr * yi + xi * yr;
-      ar1 += xr1 * yr1 - xi1 * yi1;
-      ai1 += xr1 * yi1 + xi1 * yr1;
+  if (tid < BTC_NB) {   // diagonal blocks of BT_LEAF by the recurrence
+    const int k = tid, end = (k / BT_LEAF + 1) * BT_LEAF;
+    const int tk = btc_t_idx(k, 0);
+    sTr[tk + k] = str[k];
+    sTi[tk + k] = sti[k];
+    for (int i = k + 1; i < end; ++i) {
+      const int si = btc_s_idx(0, i);
+      double ar = 0.0, ai = 0.0;
+      for (int l = k; l < i; ++l) {
+        const double xr = sTr[tk + l], xi = sTi[tk + l], yr = sSr[si + l], yi = sSi[si + l];
+        ar += xr * yr - xi * yi;
+        ai += xr * yi + xi * yr;
+      }
+      sTr[tk + i] = -(str[i] * ar - sti[i] * ai);
+      sTi[tk + i] = -(str[i] * ai + sti[i] * ar);
     }
-    if (l < i) {
-      const double xr = sTr[tk + l], xi = sTi[tk + l], yr = sSr[si + l], yi = sSi[si + l];
-      ar0 += xr * yr - xi * yi;
-      ai0 += xr * yi + xi * yr;
-    }
-    const double ar = ar0 + ar1, ai = ai0 + ai1;
-    const double tr = str[i], ti = sti[i];
-    sTr[btc_t_idx(k, i)] = -(tr * ar - ti * ai);
-    sTi[btc_t_idx(k, i)] = -(tr * ai + ti * ar);
   }
   __syncthreads();
+  constexpr int MAXPER = BTC_NB * (BTC_NB / 2) / 2 / 256;   // 4
+  for (int h = BT_LEAF; h < BTC_NB; h *= 2) {
+    const int per = BTC_NB * h / 2 / 256;
+    double xr[MAXPER], xi[MAXPER];
+#pragma unroll
+    for (int e = 0; e < MAXPER; ++e) {   // X = S12 T2 into the T12 slots
+      if (e < per) {
+        const int o = tid + 256 * e;
+        const int p = o / (h * h), r = o % (h * h);
+        const int b0 = 2 * h * p, k = b0 + r / h, i = b0 + h + r % h;
+        double ar = 0.0, ai = 0.0;
+        for (int l = b0 + h; l <= i; ++l) {
+          const double sr = sSr[btc_s_idx(k, l)], si = sSi[btc_s_idx(k, l)];
+          const double tr = sTr[btc_t_idx(l, i)], ti = sTi[btc_t_idx(l, i)];
+          ar += sr * tr - si * ti;
+          ai += sr * ti + si * tr;
+        }
+        xr[e] = ar;
+        xi[e] = ai;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < MAXPER; ++e) {
+      if (e < per) {
+        const int o = tid + 256 * e;
+        const int p = o / (h * h), r = o % (h * h);
+        const int b0 = 2 * h * p, k = b0 + r / h, i = b0 + h + r % h;
+        sTr[btc_t_idx(k, i)] = xr[e];
+        sTi[btc_t_idx(k, i)] = xi[e];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < MAXPER; ++e) {   // T12 = -T1 X
+      if (e < per) {
+        const int o = tid + 256 * e;
+        const int p = o / (h * h), r = o % (h * h);
+        const int b0 = 2 * h * p, k = b0 + r / h, i = b0 + h + r % h;
+        const int tk = btc_t_idx(k, 0);
+        double ar = 0.0, ai = 0.0;
+        for (int l = k; l < b0 + h; ++l) {
+          const double ur = sTr[tk + l], ui = sTi[tk + l];
+          const double vr = sTr[btc_t_idx(l, i)], vi = sTi[btc_t_idx(l, i)];
+          ar += ur * vr - ui * vi;
+          ai += ur * vi + ui * vr;
+        }
+        xr[e] = -ar;
+        xi[e] = -ai;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < MAXPER; ++e) {
+      if (e < per) {
+        const int o = tid + 256 * e;
+        const int p = o / (h * h), r = o % (h * h);
+        const int b0 = 2 * h * p, k = b0 + r / h, i = b0 + h + r % h;
+        sTr[btc_t_idx(k, i)] = xr[e];
+        sTi[btc_t_idx(k, i)] = xi[e];
+      }
+    }
+    __syncthreads();
+  }
   double* out = TtS + (size_t)b * BT_SPLIT * BTC_NB * BTC_NB;
-  for (int s = 0; s < BT_SPLIT; ++s)
-    for (int r = 0; r < BTC_NB; ++r) {   // row r of T^T (column r of T), coalesced over i = k
+  for (int idx = tid; idx < BTC_NB * BTC_NB; idx += 256) {   // row r of T^T (column r of T)
+    const int r = idx / BTC_NB, k = idx % BTC_NB;
+    const double vr = k <= r ? sTr[btc_t_idx(k, r)] : 0.0, vi = k <= r ? sTi[btc_t_idx(k, r)] : 0.0;
+    for (int s = 0; s < BT_SPLIT; ++s) {
       const size_t o = ((size_t)s * BTC_NB + r) * BTC_NB + k;
-      out[o] = k <= r ? sTr[btc_t_idx(k, r)] : 0.0;
-      out[o + pT] = k <= r ? sTi[btc_t_idx(k, r)] : 0.0;
+      out[o] = vr;
+      out[o + pT] = vi;
     }
+  }
 }
 
 // Et (keep x n planar, plane stride keep*n) <- rows of Q Y for hetrd_lower's
@@ -1247,7 +1365,7 @@ void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
     g.pC = (long long)nS;
     gemm_f64(c, g, Vall, Vall, Sm);
   }
-  btc_larft_k<<<nblk, NB, 0, c->stream>>>(Sm, (long long)nS, tau, nrefl, TtS, (long long)nT);
+  btc_larft_k<<<nblk, 256, 0, c->stream>>>(Sm, (long long)nS, tau, nrefl, TtS, (long long)nT);
   DMRG_CHECK_HIP(hipGetLastError());
   const long long pE = (long long)keep * n;
   for (int b = nblk - 1; b >= 0; --b) {
