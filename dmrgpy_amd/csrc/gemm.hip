@@ -186,11 +186,36 @@ __global__ __launch_bounds__(256) void gemm_mfma(const T* __restrict__ A, const 
 // Real FP64 / FP32 kernel (the H_eff and environment GEMMs): 16-byte global
 // loads when the host found base pointers, leading dimensions and batch
 // strides multiples of 16 bytes (g.vec), element loads otherwise;
-// double-buffered LDS (one barrier per k-tile) and an XCD-aware tile order.
-// Workgroups are dispatched round-robin over the 8 XCDs; the tile order is
-// remapped so each XCD owns a contiguous run of row-major tiles, i.e. the
-// tiles that share an A row panel share an L2.
+// double-buffered LDS (one barrier per k-tile) and an XCD-aware tile order
+// (xcd_tile below).
 // ---------------------------------------------------------------------------
+
+// Tile (row, column) of this workgroup.  Workgroups are dispatched
+// round-robin over the 8 XCDs (linear id lin on XCD lin % 8), so XCD x runs
+// lin = x, x + 8, ... in that order; that sequence is mapped onto a
+// contiguous run of the tiles in grouped order -- XCD_GM row tiles at a
+// time, column by column inside a group -- so the workgroups resident on
+// one XCD at the same time cover a near-square block of tiles and share both
+// their A row panels and their B column panels in that XCD's L2.  A run of
+// row-major tiles shares only the A panel: the wide L-step GEMM of H_eff
+// then re-fetched its whole B operand once per 128-row band (3.3 GB per
+// D=1024 apply, 1.1 GB in this order; group heights 4 and 8 measured
+// equal, 16 worse; kernel times unchanged within 1%).  Any tile count
+// works (the first nblk % 8 XCDs take one tile more).
+constexpr int XCD_GM = 8;
+__device__ __forceinline__ void xcd_tile(int nbx, int nby, int& tr, int& tc) {
+  const int nblk = nbx * nby;
+  const int lin = blockIdx.y * nbx + blockIdx.x;
+  const int x = lin & 7, t = lin >> 3;
+  const int per = nblk >> 3, rem = nblk & 7;
+  const int idx = x * per + min(x, rem) + t;
+  const int gsz = XCD_GM * nbx;
+  const int g = idx / gsz, first = g * XCD_GM;
+  const int rows = min(XCD_GM, nby - first);
+  const int w = idx - g * gsz;
+  tr = first + w % rows;
+  tc = w / rows;
+}
 template <typename T> struct Vec;
 template <> struct Vec<float> { typedef flt4 type; static constexpr int W = 4; };
 template <> struct Vec<double> { typedef double dbl2 __attribute__((ext_vector_type(2))); typedef dbl2 type; static constexpr int W = 2; };
@@ -219,11 +244,9 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
   A += bz * g.sA;
   B += bz * g.sB;
   C += bz * g.sC;
-  // XCD-aware tile order
-  const int nbx = gridDim.x, nblk = gridDim.x * gridDim.y;
-  int lin = blockIdx.y * nbx + blockIdx.x;
-  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
-  const int m0 = (lin / nbx) * BM, n0 = (lin % nbx) * BN;
+  int tr, tc;
+  xcd_tile(gridDim.x, gridDim.y, tr, tc);
+  const int m0 = tr * BM, n0 = tc * BN;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -553,10 +576,9 @@ __global__ __launch_bounds__(256) void gemm_b3nt_k(const __bf16* __restrict__ Ah
   constexpr int NA = BM * BK / (8 * 256), NB = BN * BK / (8 * 256); // 16-byte vectors per thread
   __shared__ __attribute__((aligned(16))) __bf16 sAh[2][BM][LDK], sAl[2][BM][LDK];
   __shared__ __attribute__((aligned(16))) __bf16 sBh[2][BN][LDK], sBl[2][BN][LDK];
-  const int nbx = gridDim.x, nblk = gridDim.x * gridDim.y;
-  int lin = blockIdx.y * nbx + blockIdx.x;
-  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
-  const int m0 = (lin / nbx) * BM, n0 = (lin % nbx) * BN;
+  int tr, tc;
+  xcd_tile(gridDim.x, gridDim.y, tr, tc);
+  const int m0 = tr * BM, n0 = tc * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
