@@ -336,6 +336,27 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
     }
   };
 
+  // FP64 64 x 64 tiles with beta != 0 (among them the tridiagonalisation's
+  // trailing updates, inner dimension 64): the C tile is loaded before the
+  // k-loop, so its read does not wait behind the MFMA work in the epilogue
+  // (the larger FP64 tile keeps its two waves per SIMD without it)
+  constexpr bool CPRE = sizeof(T) == 8 && sizeof(CT) == 8 && TM * TN <= 4;
+  CT cpre[CPRE ? TM : 1][CPRE ? TN : 1][4];
+  const bool pre = CPRE && g.beta != 0.0;
+  if (CPRE && pre) {
+#pragma unroll
+    for (int a = 0; a < (CPRE ? TM : 1); ++a)
+#pragma unroll
+      for (int b = 0; b < (CPRE ? TN : 1); ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int rr = (sizeof(T) == 8) ? (4 * r + lk) : (lk * 4 + r);
+          const int row = m0 + wm * TM * 16 + a * 16 + rr;
+          const int col = n0 + wn * TN * 16 + b * 16 + li;
+          cpre[a][b][r] = (row < M && col < N) ? C[(long long)row * g.ldc + col] : CT(0);
+        }
+  }
+
   load_regs(0);
   store_regs(0);
   __syncthreads();
@@ -380,7 +401,12 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
         if (row < M && col < N) {
           const long long off = (long long)row * g.ldc + col;
           T v = alpha * acc[a][b][r];
-          if (g.beta != 0.0) v += beta * (T)C[off];
+          if (g.beta != 0.0) {
+            T cold;
+            if constexpr (CPRE) cold = (T)cpre[CPRE ? a : 0][CPRE ? b : 0][r];
+            else cold = (T)C[off];
+            v += beta * cold;
+          }
           C[off] = (CT)v;
         }
       }
