@@ -125,10 +125,14 @@ struct Ctx {
 void* ws_get(Ctx* c, Slot s, size_t bytes);
 
 // Early stop of the Gram matrices' tridiagonalisation (eigensolver mode 1):
-// relative Frobenius norm below which the trailing block is taken as zero
-// (about one rounding of the matrix; every eigenvalue moves by at most this
-// times |G|_F)
-constexpr double TRD_STOP_TOL = 2e-16;
+// relative Frobenius norm below which the trailing block is taken as zero.
+// Every eigenvalue moves by at most this times |G|_F (about 9 eps |G|_F),
+// an order of magnitude below the rounding of forming G = M M^dag in FP64
+// (~sqrt(k) eps tr(G) in Frobenius norm for inner dimension k ~ 10^3); the
+// block's trace is added to the discarded weight, so the keep rule sees the
+// whole spectrum's weight.  2e-16 was measured too: same keep counts, ~17%
+// more reduced columns.
+constexpr double TRD_STOP_TOL = 2e-15;
 
 // ---- GEMM (gemm.hip) ------------------------------------------------------
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b], row-major, logical A is

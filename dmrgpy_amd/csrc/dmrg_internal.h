@@ -64,13 +64,16 @@ bool smallop_sparse_ok(const SmallOp& s);
 // dsytrd (uplo 'L') of a symmetric n x n matrix in place: d[n], e[n-1],
 // tau[n-1]; returns the number of reflectors computed (n - 1, or fewer when
 // stop_tol > 0 and the trailing block fell below stop_tol |A|_F: the rest of
-// d, e, tau is then zero)
-int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol = 0.0);
+// d, e, tau is then zero, and *dropped, when given, receives the trace of
+// the block taken as zero)
+int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol = 0.0,
+                double* dropped = nullptr);
 // zhetrd (uplo 'L') of the column-major view of a Hermitian n x n planar
 // complex buffer (plane stride n*n) in place: d[n], e[n-1] real, tau[n-1]
 // complex interleaved; returns the number of reflectors computed (as
 // sytrd_lower, with the same early stop over both planes)
-int hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol = 0.0);
+int hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol = 0.0,
+                double* dropped = nullptr);
 // rows of Zt (keep x n, row-major: eigenvectors of the tridiagonal) <- rows
 // of Q Y for the first nrefl reflectors sytrd_lower left in A (all n - 1 by
 // default; dormtr L/lower/N by blocked compact-WY GEMMs)

@@ -444,6 +444,7 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
   double* Gsolve = nullptr;
   double *trd_Z = nullptr, *trd_tau = nullptr, *trd_refl = nullptr, *trd_Zs = nullptr;
   int trd_nrefl = g - 1, trd_m = g;   // reflectors computed, order of the tridiagonal solved
+  double trd_dropped = 0.0;             // trace of the block a stopped reduction took as zero
   {
     ProfScope ps(c, 2, (cx ? 8.0 : 2.0) * (double)g * g * (left ? n : m));
     double* Gplanar = (double*)ws_get(c, WS_SVD_TMP2, sizeof(double) * es * (size_t)g * g);
@@ -467,7 +468,8 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
       // conquer on it; unmtr on the kept columns after keep
       trd_Z = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * ((size_t)g * g + 2 * (size_t)g + 16));
       trd_tau = trd_Z + (size_t)g * g;
-      trd_nrefl = hetrd_lower(c, Gplanar, g, ev, ework, trd_tau, c->eig_mode == 1 ? TRD_STOP_TOL : 0.0);
+      trd_nrefl = hetrd_lower(c, Gplanar, g, ev, ework, trd_tau, c->eig_mode == 1 ? TRD_STOP_TOL : 0.0,
+                              &trd_dropped);
       trd_m = trd_nrefl < g - 1 ? trd_nrefl + 1 : g;
       if (trd_m < g) {
         trd_Zs = (double*)ws_get(c, WS_TRD_ZS, sizeof(double) * (size_t)trd_m * trd_m);
@@ -494,7 +496,8 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
       // spectrum is exactly zero.
       trd_Z = (double*)ws_get(c, WS_TRD_Z, sizeof(double) * ((size_t)g * g + g + 16));
       trd_tau = trd_Z + (size_t)g * g;
-      trd_nrefl = sytrd_lower(c, Gplanar, g, ev, ework, trd_tau, c->eig_mode == 1 ? TRD_STOP_TOL : 0.0);
+      trd_nrefl = sytrd_lower(c, Gplanar, g, ev, ework, trd_tau, c->eig_mode == 1 ? TRD_STOP_TOL : 0.0,
+                              &trd_dropped);
       trd_m = trd_nrefl < g - 1 ? trd_nrefl + 1 : g;
       if (trd_m < g) {
         trd_Zs = (double*)ws_get(c, WS_TRD_ZS, sizeof(double) * (size_t)trd_m * trd_m);
@@ -526,7 +529,9 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
   svals.resize(rank);
   for (int i = 0; i < rank; ++i) svals[i] = scale * std::sqrt(std::max(evh[g - 1 - i], 0.0));
   SvdOut o;
-  o.keep = truncate_rule(svals, cutoff, maxdim, mind, &o.discarded, extra);
+  // a stopped reduction's dropped block: its eigenvalues (all below the
+  // stop tolerance) are counted as discarded weight, in units of s^2
+  o.keep = truncate_rule(svals, cutoff, maxdim, mind, &o.discarded, extra + scale * scale * trd_dropped);
   const int keep = o.keep;
   if (!(svals[0] > 0.0 && svals[keep - 1] >= 1e-7 * svals[0]) || keep > trd_m)
     return svd_full(c, field, M, m, n, cutoff, maxdim, mind, absorb, U, V, kcap, svals);

@@ -377,6 +377,28 @@ __global__ void trd_clear_tail_k(int n, int s, double* __restrict__ d, double* _
 // |A[s:, s:]|_F^2 into host_scalars[32 + slot] (pinned), with event
 // trd_ev[slot] recorded after the copy: the caller waits on the event, so a
 // check can stay in flight while the next panel is queued
+// trace of the trailing block, sum_{r >= s} A[r][r] (real plane), one
+// workgroup, fixed order: the weight a stopped reduction drops, which the
+// keep rule adds back to the discarded weight
+__global__ __launch_bounds__(256) void trd_trace_k(const double* __restrict__ A, int n, int s,
+                                                   double* __restrict__ out) {
+  __shared__ double red4[4];
+  double acc = 0.0;
+  for (int r = s + threadIdx.x; r < n; r += 256) acc += A[(size_t)r * n + r];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = red4[0] + red4[1] + red4[2] + red4[3];
+}
+
+static double trailing_trace(Ctx* c, const double* A, int n, int s, double* out) {
+  trd_trace_k<<<1, 256, 0, c->stream>>>(A, n, s, out);
+  DMRG_CHECK_HIP(hipGetLastError());
+  DMRG_CHECK_HIP(hipMemcpyAsync(c->host_scalars + 34, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return std::max(0.0, c->host_scalars[34]);
+}
+
 static void trailing_sumsq_async(Ctx* c, const double* A, int n, int s, int planes, double* part, int cap,
                                  double* out, int slot) {
   const int nb = std::max(1, std::min(std::min(TRD_SQ_BLOCKS, n - s), cap));
@@ -407,11 +429,12 @@ static double trailing_sumsq_wait(Ctx* c, int slot) {
 // passed; the trailing norm never grows (each step is an orthogonal
 // similarity of the trailing block, of which the next one is a principal
 // submatrix), so the later block passes too.  That perturbs A by at most
-// stop_tol |A|_F (every eigenvalue by at most as much), far below the
-// rounding of the reduction itself at stop_tol ~ 2e-16; the dense
-// eigenproblem then shrinks to order s + 1 plus zeros.
+// stop_tol |A|_F (every eigenvalue by at most as much); the caller's
+// stop_tol sits below the rounding of forming a Gram matrix in FP64.  The
+// dense eigenproblem then shrinks to order s + 1 plus zeros, and the trace
+// of the dropped block (its eigenvalues' sum) is returned in *dropped.
 constexpr int TRD_STOP_MIN = 256;
-int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol) {
+int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol, double* dropped) {
   DMRG_REQUIRE(n >= 2, "sytrd: order must be at least 2");
   const size_t vwv = (size_t)n * TRD_P;   // the [V | W | V] panel store
   const int gmax1 = (n + TRD_ROWS1 - 1) / TRD_ROWS1;
@@ -483,6 +506,7 @@ int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, dou
       // the previous panel's check: its block passed => this one (already
       // queued and applied) passes too; stop here
       if (pend >= 0 && trailing_sumsq_wait(c, pend) <= stop2 && s < n - 1) {
+        if (dropped) *dropped = trailing_trace(c, A, n, s, sq_out + 2);
         trd_clear_tail_k<<<(n - s + 255) / 256, 256, 0, c->stream>>>(n, s, d, e, tau, 1);
         DMRG_CHECK_HIP(hipGetLastError());
         done = s;
@@ -888,7 +912,7 @@ __global__ __launch_bounds__(256) void trc_fin_k(int n, int s, int k, const doub
 // reflectors in place (zhetrd, uplo 'L', on the column-major view).  Returns
 // the number of reflectors computed; stop_tol > 0 stops the reduction as
 // sytrd_lower does (trailing block, both planes, below stop_tol |A|_F).
-int hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol) {
+int hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol, double* dropped) {
   DMRG_REQUIRE(n >= 2, "hetrd: order must be at least 2");
   const size_t vw = (size_t)n * TRD_P;   // one plane of the [W | V | W] panel store
   const int gmax1 = (n + TRD_ROWS1 - 1) / TRD_ROWS1;
@@ -955,6 +979,7 @@ int hetrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, dou
         trailing_sumsq_async(c, A, n, s, 2, part1, sq_cap, sq_out, slot);
       }
       if (pend >= 0 && trailing_sumsq_wait(c, pend) <= stop2 && s < n - 1) {
+        if (dropped) *dropped = trailing_trace(c, A, n, s, sq_out + 2);   // real plane: Hermitian diagonal
         trd_clear_tail_k<<<(n - s + 255) / 256, 256, 0, c->stream>>>(n, s, d, e, tau, 2);
         DMRG_CHECK_HIP(hipGetLastError());
         done = s;

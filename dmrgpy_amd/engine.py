@@ -232,7 +232,7 @@ class Engine:
         """Eigensolver of the Gram-route splits.  1 (default): own
         tridiagonalisation (real or Hermitian) + rocSOLVER stedc + the
         library's blocked back-transformation of the kept eigenvectors, the
-        real reduction stopping once its trailing block is below 2e-16 of
+        reduction stopping once its trailing block is below 2e-15 of
         the Gram matrix in Frobenius norm (orders above 288); 3: as 1 with
         the complete reduction; 2: as 3 with rocSOLVER ormtr/unmtr for the
         back-transformation; 0: rocSOLVER syevd/heevd."""

@@ -173,11 +173,12 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
  * back-transforms only the eigenvectors the truncation keeps with this
  * library's blocked compact-WY GEMMs.  For a real Gram matrix of order
  * above 288, mode 1 stops the reduction after the first full panel (at or
- * past column 256) whose trailing block has Frobenius norm at most 2e-16
+ * past column 256) whose trailing block has Frobenius norm at most 2e-15
  * of the Gram matrix's: that block's eigenvalues are then taken as zero
- * (they lie below the FP64 rounding of the reduction itself) and the
- * divide and conquer runs on the reduced order; a split that would keep
- * more values than the reduced order falls back to the full SVD.  Mode 3
+ * (they lie below the FP64 rounding of forming the Gram matrix), its trace
+ * is counted as discarded weight, and the divide and conquer runs on the
+ * reduced order; a split that would keep more values than the reduced
+ * order falls back to the full SVD.  Mode 3
  * is mode 1 with the complete reduction; mode 2 is mode 3 with rocSOLVER's
  * dormtr / zunmtr for the back-transformation; mode 0 calls rocSOLVER
  * dsyevd / zheevd. */

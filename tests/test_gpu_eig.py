@@ -412,7 +412,7 @@ def test_split_early_stop_matches_full_reduction(engine, field, shape):
     falling by 1e-10 across the rows / columns, as Schmidt weights of the
     neighbouring bonds grade a DMRG theta), so the Gram matrix's rounding
     noise is graded too and its trailing blocks fall far below eps |G|_F:
-    the reduction of mode 1 stops once the trailing block is below 2e-16 of
+    the reduction of mode 1 stops once the trailing block is below 2e-15 of
     |G|_F (at about half the order here, CPU Householder); the split must
     then agree with the complete reduction (mode 3) -- keep count,
     discarded weight to 1e-14, kept singular values and the truncated
