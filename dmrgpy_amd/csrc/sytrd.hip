@@ -433,7 +433,7 @@ static double trailing_sumsq_wait(Ctx* c, int slot) {
 // stop_tol sits below the rounding of forming a Gram matrix in FP64.  The
 // dense eigenproblem then shrinks to order s + 1 plus zeros, and the trace
 // of the dropped block (its eigenvalues' sum) is returned in *dropped.
-constexpr int TRD_STOP_MIN = 256;
+constexpr int TRD_STOP_MIN = 64;
 int sytrd_lower(Ctx* c, double* A, int n, double* d, double* e, double* tau, double stop_tol, double* dropped) {
   DMRG_REQUIRE(n >= 2, "sytrd: order must be at least 2");
   const size_t vwv = (size_t)n * TRD_P;   // the [V | W | V] panel store

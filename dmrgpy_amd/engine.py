@@ -233,7 +233,7 @@ class Engine:
         tridiagonalisation (real or Hermitian) + rocSOLVER stedc + the
         library's blocked back-transformation of the kept eigenvectors, the
         reduction stopping once its trailing block is below 2e-15 of
-        the Gram matrix in Frobenius norm (orders above 288); 3: as 1 with
+        the Gram matrix in Frobenius norm (orders above 96); 3: as 1 with
         the complete reduction; 2: as 3 with rocSOLVER ormtr/unmtr for the
         back-transformation; 0: rocSOLVER syevd/heevd."""
         check(self.lib.dmrg_set_eigensolver(self.ctx, int(mode)), "dmrg_set_eigensolver")

@@ -172,8 +172,8 @@ int dmrg_svd_split_guided(dmrg_ctx* ctx, int field, const double* M, int m, int 
  * kernels, runs rocSOLVER's divide and conquer on the tridiagonal and
  * back-transforms only the eigenvectors the truncation keeps with this
  * library's blocked compact-WY GEMMs.  For a real Gram matrix of order
- * above 288, mode 1 stops the reduction after the first full panel (at or
- * past column 256) whose trailing block has Frobenius norm at most 2e-15
+ * above 96, mode 1 stops the reduction after the first full panel (at or
+ * past column 64) whose trailing block has Frobenius norm at most 2e-15
  * of the Gram matrix's: that block's eigenvalues are then taken as zero
  * (they lie below the FP64 rounding of forming the Gram matrix), its trace
  * is counted as discarded weight, and the divide and conquer runs on the

@@ -466,9 +466,8 @@ def test_split_early_stop_matches_full_reduction(engine, field, shape):
     # the reduction stopped early: at ~45% of the order on the large cases,
     # which saves 1 - (1 - 0.45)^3 ~ 16% of its bytes (the first columns
     # stream the largest trailing blocks) and ~40% of its time (two
-    # launches per column, latency bound); the first check comes at column
-    # 256, so the small case saves only its last columns
-    assert be < (0.9 if m >= 1000 else 1.0) * bf
+    # launches per column, latency bound)
+    assert be < 0.95 * bf
 
 
 @pytest.mark.parametrize("field", ["real", "auto"])
