@@ -291,15 +291,28 @@ SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& w
 
 // ---------------------------------------------------------------------------
 // Sparse small operator for a real W: out[x][o][b] = sum over the nonzeros
-// (i, w) of row o of  w * in[x][i][b].  One block per (x, 512 inner
-// elements, plane); eight chunks of 64 columns are staged in LDS in turn
+// (i, w) of row o of  w * in[x][i][b].  One block per (x, up to
+// SP_CHUNKS chunks of 64 inner elements, plane); the chunks are staged in LDS in turn
 // (prefetched into registers one chunk ahead), each lane owns one column and
 // each wave a contiguous range of output rows with about a quarter of the
 // nonzeros.  Row bounds, column indices and values are uniform across a wave
 // (scalar loads); every output row is written as one 256-byte segment per
 // wave.  HBM traffic: the input once and the output once.
 // ---------------------------------------------------------------------------
-constexpr int SP_BN = 64, SP_LDI = SP_BN + 1, SP_CHUNKS = 8, SP_KMAX = 128, SP_PRE = SP_KMAX * SP_BN / 256;
+constexpr int SP_BN = 64, SP_LDI = SP_BN + 1, SP_CHUNKS = 2, SP_KMAX = 128, SP_PRE = SP_KMAX * SP_BN / 256;
+
+// Blocks along the inner index and chunks of SP_BN columns per block: at
+// most SP_CHUNKS chunks, spread evenly (a bond dimension just above a
+// multiple of SP_CHUNKS * SP_BN would otherwise leave every second block
+// with one chunk of work).  Measured on the FP32 Hubbard apply at D = 500 …
+// 1024: at most 2 chunks per block (more blocks in flight) 1-4% faster per
+// apply at D = 500 … 749 than the former 8 unbalanced, 4 or 8 balanced in
+// between, equal at 800 … 1024.
+static void sp_split(int inner, int& nbx, int& cpb) {
+  const int nch = (inner + SP_BN - 1) / SP_BN;
+  nbx = (nch + SP_CHUNKS - 1) / SP_CHUNKS;
+  cpb = (nch + nbx - 1) / nbx;
+}
 
 template <typename T, bool SPLIT>
 __global__ __launch_bounds__(256) void smallop_sparse_k(const T* __restrict__ in, T* __restrict__ out,
@@ -308,7 +321,8 @@ __global__ __launch_bounds__(256) void smallop_sparse_k(const T* __restrict__ in
                                                         const int* __restrict__ grp, const int* __restrict__ ioff,
                                                         const int* __restrict__ ooff, int inner, long long sx_in,
                                                         long long sx_out, int Kin, long long pin, long long pout,
-                                                        __bf16* __restrict__ out_hi, __bf16* __restrict__ out_lo) {
+                                                        __bf16* __restrict__ out_hi, __bf16* __restrict__ out_lo,
+                                                        int cpb) {
   extern __shared__ double smem_d[];
   T* Is = reinterpret_cast<T*>(smem_d);                 // [Kin][SP_LDI]
   int* io = reinterpret_cast<int*>(Is + Kin * SP_LDI);  // [Kin]
@@ -333,10 +347,10 @@ __global__ __launch_bounds__(256) void smallop_sparse_k(const T* __restrict__ in
       pre[u] = (i < Kin && b < inner) ? in[(long long)io[i] + b] : T(0);
     }
   };
-  const int bfirst = blockIdx.x * SP_CHUNKS * SP_BN;
+  const int bfirst = blockIdx.x * cpb * SP_BN;
   __syncthreads();   // offsets ready
   fetch(bfirst);
-  for (int ch = 0; ch < SP_CHUNKS; ++ch) {
+  for (int ch = 0; ch < cpb; ++ch) {
     const int b0 = bfirst + ch * SP_BN;
     if (b0 >= inner) break;
     __syncthreads();   // previous chunk's LDS reads done
@@ -346,7 +360,7 @@ __global__ __launch_bounds__(256) void smallop_sparse_k(const T* __restrict__ in
       if (e < Kin * SP_BN) Is[(e / SP_BN) * SP_LDI + e % SP_BN] = pre[u];
     }
     __syncthreads();
-    if (ch + 1 < SP_CHUNKS && b0 + SP_BN < inner) fetch(b0 + SP_BN);
+    if (ch + 1 < cpb && b0 + SP_BN < inner) fetch(b0 + SP_BN);
     const int b = b0 + lane;
     for (int o = r0; o < r1; ++o) {
       T acc = T(0);
@@ -480,11 +494,13 @@ void run_smallop_split(Ctx* c, const SmallOp& s, const float* in, __bf16* hi, __
   if (X <= 0 || inner <= 0 || s.Kout <= 0) return;
   DMRG_REQUIRE(X <= 65535, "smallop: outer extent too large");
   DMRG_REQUIRE(smallop_sparse_ok(s), "smallop: split output needs the sparse real pass");
-  dim3 g((inner + SP_BN * SP_CHUNKS - 1) / (SP_BN * SP_CHUNKS), X, 1);
+  int nbx, cpb;
+  sp_split(inner, nbx, cpb);
+  dim3 g(nbx, X, 1);
   const size_t lds = sizeof(float) * (size_t)s.Kin * SP_LDI + sizeof(int) * s.Kin;
   smallop_sparse_k<float, true><<<g, 256, lds, c->stream>>>(in, nullptr, s.rowptr, s.col, (const float*)s.val, s.grp,
                                                             s.ioff, s.ooff, inner, sx_in, sx_out, s.Kin, 0, 0, hi,
-                                                            lo);
+                                                            lo, cpb);
   DMRG_CHECK_HIP(hipGetLastError());
 }
 
@@ -494,17 +510,21 @@ void run_smallop(Ctx* c, int prec, int field, const SmallOp& s, const void* in, 
   DMRG_REQUIRE(X <= 65535, "smallop: outer extent too large");
   const bool cv = field == CPLX;
   if (smallop_sparse_ok(s)) {
-    dim3 g((inner + SP_BN * SP_CHUNKS - 1) / (SP_BN * SP_CHUNKS), X, cv ? 2 : 1);
+    int nbx, cpb;
+    sp_split(inner, nbx, cpb);
+    dim3 g(nbx, X, cv ? 2 : 1);
     const size_t es = prec_size(prec);
     const size_t lds = es * (size_t)s.Kin * SP_LDI + sizeof(int) * s.Kin;
     if (prec == P64)
       smallop_sparse_k<double, false><<<g, 256, lds, c->stream>>>((const double*)in, (double*)out, s.rowptr, s.col,
                                                                   (const double*)s.val, s.grp, s.ioff, s.ooff, inner,
-                                                                  sx_in, sx_out, s.Kin, pin, pout, nullptr, nullptr);
+                                                                  sx_in, sx_out, s.Kin, pin, pout, nullptr, nullptr,
+                                                                  cpb);
     else
       smallop_sparse_k<float, false><<<g, 256, lds, c->stream>>>((const float*)in, (float*)out, s.rowptr, s.col,
                                                                  (const float*)s.val, s.grp, s.ioff, s.ooff, inner,
-                                                                 sx_in, sx_out, s.Kin, pin, pout, nullptr, nullptr);
+                                                                 sx_in, sx_out, s.Kin, pin, pout, nullptr, nullptr,
+                                                                 cpb);
     DMRG_CHECK_HIP(hipGetLastError());
     return;
   }
