@@ -93,6 +93,51 @@ print(json.dumps(out))
 """
 
 
+CPU_BOND = r"""
+import json, sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle import dmrg as od, linalg as ol, mps as om
+spec = json.loads(sys.argv[2])
+Dl, Dr, d, w, maxdim = spec["Dl"], spec["Dr"], spec["d"], spec["w"], spec["maxdim"]
+rng = np.random.default_rng(0)
+c = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
+L, R = c(Dl, w, Dl), c(Dr, w, Dr)
+W1, W2 = c(w, d, d, w), c(w, d, d, w)
+th = c(Dl, d, d, Dr)
+A = c(Dl, d, Dr)
+mv2 = od.heff2_matvec(L, W1, W2, R, th.shape)
+mv1 = od.heff1_matvec(L, W1, R, A.shape)
+x2, x1 = th.reshape(-1), A.reshape(-1)
+mv2(x2)
+mv1(x1)                                     # warm-up (BLAS threads, page faults)
+t = {}
+t0 = time.perf_counter(); mv2(x2); t["heff2"] = time.perf_counter() - t0
+t0 = time.perf_counter(); mv1(x1); t["heff1"] = time.perf_counter() - t0
+t0 = time.perf_counter(); ol.svd_truncated(th.reshape(Dl * d, d * Dr), 1e-12, maxdim, 1)
+t["svd"] = time.perf_counter() - t0
+t0 = time.perf_counter(); om.extend_left(L, A, W1, A); t["env"] = time.perf_counter() - t0
+print(json.dumps(t))
+"""
+
+
+def cpu_bond_sample(Dl, Dr, d, w, maxdim, threads):
+    """One bond of the oracle (NumPy complex128, the reference algorithm):
+    a two-site and a one-site H_eff matvec, the truncated SVD of theta and
+    an environment update, timed once each after a warm-up, on `threads`
+    BLAS threads (set before numpy is imported)."""
+    env = dict(os.environ)
+    for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS"):
+        env[k] = str(threads)
+    spec = {"Dl": Dl, "Dr": Dr, "d": d, "w": w, "maxdim": maxdim}
+    p = subprocess.run([sys.executable, "-c", CPU_BOND, ROOT, json.dumps(spec)], env=env,
+                       capture_output=True, text=True, timeout=1800)
+    if p.returncode != 0:
+        log("cpu bond sample failed:", p.stderr[-2000:])
+        return None
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
 def stratified_bonds(shapes, iters, strata=4):
     """Bond shapes of the GPU's timed sweep grouped into `strata` groups of
     equal bond count by cost (Gram order min(Dl d1, d2 Dr), then Dl Dr); the
@@ -321,6 +366,25 @@ def workload_tdvp(args):
     dt_s = (time.perf_counter() - ts) / args.steps
     tms, flops, calls = eng.prof_read()
     eng.prof_enable(False)
+    kry = sum(stats.krylov_iters) / max(1, len(stats.krylov_iters))
+    cpu = None
+    if not args.no_cpu:
+        # the oracle on one bond at the bond-dimension cap (every bulk bond of
+        # a saturated step), scaled to a step: 2(N-1) two-site evolutions,
+        # splits and environment updates, 2(N-2) one-site evolutions, each
+        # evolution the measured mean Krylov matvec count
+        t = cpu_bond_sample(D, D, 3, 5, D, args.cpu_threads)
+        if t is not None:
+            step = 2 * (n - 1) * (kry * t["heff2"] + t["svd"] + t["env"]) + 2 * (n - 2) * kry * t["heff1"]
+            cpu = {"value": 1.0 / step, "unit": "steps/s", "cores": args.cpu_threads, "kind": "port",
+                   "sample": ("oracle (NumPy complex128, the reference algorithm) on one bulk bond at Dl = Dr = {} "
+                              "(d=3, w=5): two-site matvec {:.2f}s, one-site matvec {:.2f}s, truncated SVD {:.2f}s, "
+                              "environment update {:.2f}s, scaled to a step with every bond at the cap "
+                              "(2(N-1) splits, {:.1f} Krylov matvecs per evolution as measured on the GPU): {:.0f}s "
+                              "per step on {} BLAS threads (one run; host os.cpu_count()={})").format(
+                                  D, t["heff2"], t["heff1"], t["svd"], t["env"], kry, step, args.cpu_threads,
+                                  os.cpu_count())}
+            log("cpu baseline (TDVP step at the cap): {}".format(cpu))
     out = {"metric": "two-site TDVP step time after a local quench", "value": 1.0 / dt_s,
            "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": dt_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -331,12 +395,12 @@ def workload_tdvp(args):
                       "N": n, "D": D, "E0": e0, "svd": args.svd},
            "svd_guided_fraction": sum(stats.svd_guided) / max(1, len(stats.svd_guided)),
            "max_bond": max(psi1.bond_dims()),
-           "krylov_iters_per_call": sum(stats.krylov_iters) / max(1, len(stats.krylov_iters)),
+           "krylov_iters_per_call": kry,
            "time_split_s_per_step": {"heff": tms[0] / 1e3 / args.steps, "env": tms[1] / 1e3 / args.steps,
                                      "svd": tms[2] / 1e3 / args.steps},
            "heff_tflops": flops[0] / (tms[0] * 1e-3) / 1e12 if tms[0] > 0 else 0.0,
            "correlator": [[c.real, c.imag] for c in corr],
-           "total_s": dt_s * args.steps}
+           "total_s": dt_s * args.steps, "cpu_baseline": cpu}
     print(json.dumps(out))
 
 
@@ -460,6 +524,24 @@ def workload_gradients(args):
            "config": {"workload": "random-bond XXZ chain N={}, D={}, {} couplings".format(n, D, 2 * (n - 1)),
                       "N": n, "D": D},
            "ground_state_s": t_gs, "energy": e, "fd_checks": checks}
+    if not args.no_cpu:
+        # the oracle on one bulk bond at the cap: the gradient pass is one
+        # left and one right environment sweep plus one local contraction
+        # per site (~3N environment updates); a DMRG sweep is 2(N-1) bonds of
+        # 4 Lanczos matvecs, a split and an environment update
+        t = cpu_bond_sample(D, D, 2, 5, D, args.cpu_threads)
+        if t is not None:
+            grad_s = 3 * n * t["env"]
+            sweep_s = 2 * (n - 1) * (4 * t["heff2"] + t["svd"] + t["env"])
+            out["cpu_baseline"] = {
+                "value": 1.0 / grad_s, "unit": "gradient passes/s", "cores": args.cpu_threads, "kind": "port",
+                "sample": ("oracle (NumPy complex128, the reference algorithm) on one bulk bond at Dl = Dr = {} (d=2, "
+                           "w=5): environment update {:.3f}s, two-site matvec {:.3f}s, truncated SVD {:.3f}s; gradient "
+                           "pass = 3N environment updates = {:.2f}s, DMRG sweep = 2(N-1) x (4 matvecs + SVD + "
+                           "environment) = {:.1f}s, on {} BLAS threads (one run)").format(
+                               D, t["env"], t["heff2"], t["svd"], grad_s, sweep_s, args.cpu_threads),
+                "dmrg_sweep_s": sweep_s}
+            log("cpu baseline (gradients): {}".format(out["cpu_baseline"]))
     print(json.dumps(out))
 
 
