@@ -3,7 +3,7 @@
 Each test runs one configuration of the benchmark plan end to end through
 the session and pins it with what the domain fixes at that size: the FP64
 path against the reduced-precision path from the same seeded state (the
-north-star bound |dE|/N < 1e-9), isometry of every tensor a sweep leaves
+reduced-precision target |dE|/N < 1e-9), isometry of every tensor a sweep leaves
 behind, closed forms (free fermions), Hellmann-Feynman values computed with
 the reference's own ``vev`` semantics (src/dmrgpy/pyitensor/chain.py:277-287)
 and central finite differences.

@@ -113,7 +113,7 @@ def _heisenberg32_energies(schedules, seed=1):
 def test_fp32_vs_fp64():
     """Six FP32-contraction sweeps (FP64 Lanczos/SVD), final FP64 Rayleigh
     quotient, against six FP64 sweeps from the same seeded MPS: the
-    north-star bound |dE/N| < 1e-9."""
+    reduced-precision target |dE/N| < 1e-9."""
     n, e = _heisenberg32_energies({"fp64": ["fp64"] * 6, "fp32": ["fp32"] * 6})
     err = abs(e["fp32"] - e["fp64"]) / n
     print("fp32 vs fp64 |dE|/N", err, flush=True)
