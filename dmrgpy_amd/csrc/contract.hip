@@ -582,6 +582,7 @@ GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int ta
   g.alpha = 1.0; g.beta = 0.0;
   g.field = field;
   g.tag = tag;
+  g.lower = false;
   return g;
 }
 

@@ -152,8 +152,15 @@ struct GemmDesc {
   double alpha, beta;
   int field;
   int tag;
+  // C Hermitian by construction (A = B^dag products such as a Gram matrix):
+  // only the tiles on or below the diagonal are computed; the caller
+  // mirrors the strict upper triangle (mirror_lower)
+  bool lower = false;
 };
 void gemm_f64(Ctx* c, const GemmDesc& g, const double* A, const double* B, double* C);
+// the strict upper triangle of a Hermitian n x n result from its lower one
+// (after a GemmDesc::lower product); plane: imaginary-plane stride
+void mirror_lower(Ctx* c, double* C, int n, int ld, long long plane, int field);
 void gemm_f32(Ctx* c, const GemmDesc& g, const float* A, const float* B, float* C);
 // real FP32 operands, FP64 result (beta must be 0)
 void gemm_f32_c64(Ctx* c, const GemmDesc& g, const float* A, const float* B, double* C);

@@ -38,7 +38,8 @@ struct GemmArgs {
   long long sA, sB, sC, pA, pB, pC;
   double alpha, beta;
   int conjA, conjB;
-  int vec;   // 16-byte vector loads allowed (aligned pointers and strides)
+  int vec;     // 16-byte vector loads allowed (aligned pointers and strides)
+  int lower;   // skip tiles strictly above the diagonal (Hermitian C)
 };
 
 // ---------------------------------------------------------------------------
@@ -61,6 +62,7 @@ __global__ __launch_bounds__(256) void gemm_mfma(const T* __restrict__ A, const 
   B += bz * g.sB;
   C += bz * g.sC;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  if (g.lower && n0 >= m0 + BM) return;   // strictly above the diagonal (whole block)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
@@ -245,7 +247,16 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
   B += bz * g.sB;
   C += bz * g.sC;
   int tr, tc;
-  xcd_tile(gridDim.x, gridDim.y, tr, tc);
+  if (g.lower) {
+    // Hermitian C: row-major tiles (every XCD gets an even share of the
+    // lower-triangle tiles), those strictly above the diagonal exit at once
+    const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+    tr = lin / gridDim.x;
+    tc = lin % gridDim.x;
+    if (tc * BN >= (tr + 1) * BM) return;
+  } else {
+    xcd_tile(gridDim.x, gridDim.y, tr, tc);
+  }
   const int m0 = tr * BM, n0 = tc * BN;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -752,6 +763,7 @@ static GemmArgs make_args(const GemmDesc& d) {
   g.alpha = d.alpha; g.beta = d.beta;
   g.conjA = d.conjA; g.conjB = d.conjB;
   g.vec = 0;
+  g.lower = d.lower ? 1 : 0;
   return g;
 }
 
@@ -858,6 +870,38 @@ static void gemm_any(Ctx* c, const GemmDesc& d, const T* A, const T* B, T* C) {
   long long tiles64 = (long long)((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch;
   if (tiles64 >= 512) launch_tt<T, true, 2, 2>(c, d, A, B, C);
   else launch_tt<T, true, 1, 1>(c, d, A, B, C);
+}
+
+// C[i][j] = conj(C[j][i]) for j > i (n x n, row pitch ld, imaginary plane at
+// +plane when cx): the strict upper triangle of a Hermitian product computed
+// with GemmDesc::lower.  One 32 x 32 tile pair per block: the lower tile
+// (J, I) is read coalesced, transposed through LDS and written as (I, J).
+__global__ __launch_bounds__(256) void mirror_lower_k(double* __restrict__ C, int n, int ld, long long plane,
+                                                      int cx) {
+  __shared__ double t[32][33];
+  const int I = blockIdx.y, J = blockIdx.x;   // destination tile (I, J), I <= J
+  if (I > J) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+  for (int pl = 0; pl < (cx ? 2 : 1); ++pl) {
+    double* P = C + pl * plane;
+    for (int r = ty; r < 32; r += 8) {   // source rows J*32 + r, columns I*32 + tx
+      const int sr = J * 32 + r, sc = I * 32 + tx;
+      t[r][tx] = (sr < n && sc < n) ? P[(size_t)sr * ld + sc] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {   // destination rows I*32 + r, columns J*32 + tx
+      const int dr = I * 32 + r, dc = J * 32 + tx;
+      if (dr < n && dc < n && dc > dr) P[(size_t)dr * ld + dc] = pl ? -t[tx][r] : t[tx][r];
+    }
+    __syncthreads();
+  }
+}
+
+void mirror_lower(Ctx* c, double* C, int n, int ld, long long plane, int field) {
+  if (n <= 1) return;
+  const int nt = (n + 31) / 32;
+  mirror_lower_k<<<dim3(nt, nt), 256, 0, c->stream>>>(C, n, ld, plane, field == CPLX ? 1 : 0);
+  DMRG_CHECK_HIP(hipGetLastError());
 }
 
 void gemm_f64(Ctx* c, const GemmDesc& d, const double* A, const double* B, double* C) {

@@ -461,7 +461,9 @@ static SvdOut svd_gram(Ctx* c, int field, const double* M, int m, int n, double 
       d.lda = n; d.ldb = n; d.transA = true; d.conjA = true;
     }
     d.pA = mn; d.pB = mn; d.pC = (long long)g * g;
+    d.lower = true;   // Hermitian: the lower triangle, then its mirror
     gemm_f64(c, d, Ms, Ms, Gplanar);
+    mirror_lower(c, Gplanar, g, g, (long long)g * g, field);
     if (cx && c->eig_mode >= 1) {
       // complex Hermitian: own reduction to a real tridiagonal (stopping
       // early in mode 1 as the real one does), rocSOLVER's divide and
