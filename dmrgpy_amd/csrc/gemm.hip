@@ -12,6 +12,7 @@
 // 32 (BF16, 8 per slot) distinct k values as long as A and B agree, so each
 // lane reads 4 (resp. 8) consecutive k from LDS in one vector access.
 #include "dmrg_common.h"
+#include "tile_order.h"
 
 namespace dmrg {
 
@@ -62,7 +63,7 @@ __global__ __launch_bounds__(256) void gemm_mfma(const T* __restrict__ A, const 
   B += bz * g.sB;
   C += bz * g.sC;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  if (g.lower && n0 >= m0 + BM) return;   // strictly above the diagonal (whole block)
+  if (g.lower && block_above_diagonal(m0, n0, BM)) return;   // whole block above the diagonal
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
@@ -189,34 +190,13 @@ __global__ __launch_bounds__(256) void gemm_mfma(const T* __restrict__ A, const 
 // loads when the host found base pointers, leading dimensions and batch
 // strides multiples of 16 bytes (g.vec), element loads otherwise;
 // double-buffered LDS (one barrier per k-tile) and an XCD-aware tile order
-// (xcd_tile below).
+// (xcd_tile, tile_order.h).
 // ---------------------------------------------------------------------------
 
-// Tile (row, column) of this workgroup.  Workgroups are dispatched
-// round-robin over the 8 XCDs (linear id lin on XCD lin % 8), so XCD x runs
-// lin = x, x + 8, ... in that order; that sequence is mapped onto a
-// contiguous run of the tiles in grouped order -- XCD_GM row tiles at a
-// time, column by column inside a group -- so the workgroups resident on
-// one XCD at the same time cover a near-square block of tiles and share both
-// their A row panels and their B column panels in that XCD's L2.  A run of
-// row-major tiles shares only the A panel: the wide L-step GEMM of H_eff
-// then re-fetched its whole B operand once per 128-row band (3.3 GB per
-// D=1024 apply, 1.1 GB in this order; group heights 4 and 8 measured
-// equal, 16 worse; kernel times unchanged within 1%).  Any tile count
-// works (the first nblk % 8 XCDs take one tile more).
-constexpr int XCD_GM = 8;
+// Tile (row, column) of this workgroup in the XCD-grouped order of
+// tile_order.h.
 __device__ __forceinline__ void xcd_tile(int nbx, int nby, int& tr, int& tc) {
-  const int nblk = nbx * nby;
-  const int lin = blockIdx.y * nbx + blockIdx.x;
-  const int x = lin & 7, t = lin >> 3;
-  const int per = nblk >> 3, rem = nblk & 7;
-  const int idx = x * per + min(x, rem) + t;
-  const int gsz = XCD_GM * nbx;
-  const int g = idx / gsz, first = g * XCD_GM;
-  const int rows = min(XCD_GM, nby - first);
-  const int w = idx - g * gsz;
-  tr = first + w % rows;
-  tc = w / rows;
+  xcd_tile_of(blockIdx.x, blockIdx.y, nbx, nby, tr, tc);
 }
 template <typename T> struct Vec;
 template <> struct Vec<float> { typedef flt4 type; static constexpr int W = 4; };
@@ -253,7 +233,7 @@ __global__ __launch_bounds__(256) void gemm_fast(const T* __restrict__ A, const 
     const int lin = blockIdx.y * gridDim.x + blockIdx.x;
     tr = lin / gridDim.x;
     tc = lin % gridDim.x;
-    if (tc * BN >= (tr + 1) * BM) return;
+    if (tile_above_diagonal(tr, tc, BM, BN)) return;
   } else {
     xcd_tile(gridDim.x, gridDim.y, tr, tc);
   }
