@@ -590,8 +590,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only (DMRG sweeps are sequential over bonds; H_eff would "
-              "shard over MPO channels with one all-reduce of y per matvec); --gpus {} is not "
-              "supported".format(args.gpus))
+              "shard over the row index of the left environment with one all-gather of the Krylov "
+              "vector per matvec); --gpus {} is not supported".format(args.gpus))
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy, max(1, args.steps))
