@@ -374,9 +374,6 @@ __global__ void trd_clear_tail_k(int n, int s, double* __restrict__ d, double* _
   }
 }
 
-// |A[s:, s:]|_F^2 into host_scalars[32 + slot] (pinned), with event
-// trd_ev[slot] recorded after the copy: the caller waits on the event, so a
-// check can stay in flight while the next panel is queued
 // trace of the trailing block, sum_{r >= s} A[r][r] (real plane), one
 // workgroup, fixed order: the weight a stopped reduction drops, which the
 // keep rule adds back to the discarded weight
@@ -399,6 +396,9 @@ static double trailing_trace(Ctx* c, const double* A, int n, int s, double* out)
   return std::max(0.0, c->host_scalars[34]);
 }
 
+// |A[s:, s:]|_F^2 into host_scalars[32 + slot] (pinned), with event
+// trd_ev[slot] recorded after the copy: the caller waits on the event, so a
+// check can stay in flight while the next panel is queued
 static void trailing_sumsq_async(Ctx* c, const double* A, int n, int s, int planes, double* part, int cap,
                                  double* out, int slot) {
   const int nb = std::max(1, std::min(std::min(TRD_SQ_BLOCKS, n - s), cap));
