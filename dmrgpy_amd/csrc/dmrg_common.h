@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include "dmrg_error.h"
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -21,13 +23,6 @@ namespace dmrg {
 
 enum Field { REAL = 0, CPLX = 1 };
 enum Prec { P64 = 0, P32 = 1, PB3 = 2 };
-
-// Thrown by host code inside the library; every extern "C" entry point
-// catches it and turns it into a status code plus dmrg_last_error().
-struct Error {
-  std::string msg;
-  explicit Error(std::string m) : msg(std::move(m)) {}
-};
 
 #define DMRG_CHECK_HIP(expr)                                                  \
   do {                                                                        \
