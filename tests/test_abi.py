@@ -1,6 +1,8 @@
 """The C ABI boundary (CPU only): the shared library loads, exports every
 entry point include/dmrg_hip.h declares, and the Python binding declares
-exactly that set.  No compute call is made here (there is no GPU)."""
+exactly that set, each with the argument list of its C prototype; the
+descriptor structs have the C layout.  No compute call is made here (there
+is no GPU)."""
 
 import ctypes
 import os
@@ -27,6 +29,69 @@ def test_library_exports_header_symbols():
 
 def test_binding_matches_header():
     assert sorted(_lib.SIGNATURES) == header_functions()
+
+
+def header_prototypes():
+    """{name: (return type, [parameter types])} of every entry point, as
+    written in include/dmrg_hip.h (comments removed, names dropped)."""
+    text = open(os.path.join(ROOT, "include", "dmrg_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    protos = {}
+    for ret, name, params in re.findall(r"([A-Za-z_][\w\s\*]*?)\b(dmrg_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        ps = [] if params.strip() in ("", "void") else [q.strip() for q in params.split(",")]
+        # drop the parameter name: the last identifier of each declaration
+        types = [re.sub(r"\s*\b[A-Za-z_]\w*$", "", q) if re.search(r"[\w\*]\s+\**[A-Za-z_]\w*$", q) else q
+                 for q in ps]
+        protos[name] = (" ".join(ret.split()), [" ".join(t.replace("*", " * ").split()) for t in types])
+    return protos
+
+
+def _kind_of_c(t):
+    """Coarse class of a C type: the pointee for pointers, else the scalar."""
+    t = t.replace("const ", "").strip()
+    if t.endswith("*"):
+        base = t[:-1].strip()
+        if base.endswith("*"):
+            return ("ptr", "ptr")
+        return ("ptr", {"double": "double", "int": "int", "long long": "ll", "char": "char",
+                        "dmrg_heff_desc": "heff", "dmrg_env_desc": "env", "void": "void",
+                        "dmrg_ctx": "ctx"}[base])
+    return ("val", {"int": "int", "double": "double", "long long": "ll", "void": "void"}[t])
+
+
+def _kind_of_ctypes(ct):
+    if ct is ctypes.c_void_p:
+        return ("ptr", None)                 # any pointer (device or opaque)
+    if ct is ctypes.c_char_p:
+        return ("ptr", "char")
+    scal = {ctypes.c_int: "int", ctypes.c_double: "double", ctypes.c_longlong: "ll"}
+    if ct in scal:
+        return ("val", scal[ct])
+    target = getattr(ct, "_type_", None)
+    named = {ctypes.c_double: "double", ctypes.c_int: "int", ctypes.c_longlong: "ll", ctypes.c_void_p: "ptr",
+             _lib.HeffDesc: "heff", _lib.EnvDesc: "env"}
+    return ("ptr", named[target])
+
+
+def test_binding_argtypes_match_prototypes():
+    """Every entry point's ctypes restype / argtypes agree with its C
+    prototype: same count and order; ints, doubles and long longs where
+    the header has them; a pointer wherever the header has one, and for
+    typed ctypes pointers the same pointee (c_void_p stands for device and
+    opaque pointers)."""
+    protos = header_prototypes()
+    assert sorted(protos) == header_functions()
+    for name, (res, args) in _lib.SIGNATURES.items():
+        ret, params = protos[name]
+        assert len(args) == len(params), (name, len(args), params)
+        for i, (ct, t) in enumerate(zip(args, params)):
+            kc, kt = _kind_of_ctypes(ct), _kind_of_c(t)
+            assert kc[0] == kt[0], (name, i, t, ct)
+            if kc[1] is not None:
+                assert kc[1] == kt[1], (name, i, t, ct)
+        kr = _kind_of_c(ret)
+        assert _kind_of_ctypes(res) == kr or (kr[0] == "ptr" and _kind_of_ctypes(res)[0] == "ptr"), (name, ret)
 
 
 def test_struct_layouts():
