@@ -349,7 +349,10 @@ static SvdOut svd_full(Ctx* c, int field, const double* M, int m, int n, double 
   DMRG_CHECK_HIP(hipMemcpyAsync(S, svals.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
   DMRG_CHECK_HIP(hipStreamSynchronize(c->stream));   // svals belongs to the caller: consume it now
   SvdOut o;
-  o.keep = truncate_rule(svals, cutoff, maxdim, mindim, &o.discarded, 0.0);
+  // at most the k computed singular triplets: the rule keeps max(1, mindim)
+  // of an all-zero spectrum, which the reference's slicing U[:, :keep]
+  // (src/dmrgpy/pyitensor/svd.py:148-151) also cuts to k columns
+  o.keep = std::min(truncate_rule(svals, cutoff, maxdim, mindim, &o.discarded, 0.0), k);
   DMRG_REQUIRE(o.keep <= kcap, "svd: keep exceeds output capacity");
   const int keep = o.keep;
   // U_M (row-major m x k) is the Vx buffer, Vh_M (k x n) is the Ux buffer.

@@ -134,7 +134,9 @@ int dmrg_krylov_expm(dmrg_ctx* ctx, const dmrg_heff_desc* op, double* x, double 
  * V = S*Vh (a right-moving step); absorb 1: U*S, V = Vh (left-moving);
  * absorb 2: U, Vh with S only returned.  U is written [m][keep], V
  * [keep][n] (planar), both dense; kcap bounds keep (buffers must hold
- * m*kcap and kcap*n elements per plane).  svals_host (length min(m,n), may
+ * m*kcap and kcap*n elements per plane).  keep <= min(m, n): an all-zero M
+ * with mindim above min(m, n) keeps min(m, n), the columns the reference's
+ * slicing leaves.  svals_host (length min(m,n), may
  * be NULL) receives all singular values, descending.  Replaces svd()
  * (svd.py:154-195) as called by _apply_local_update
  * (src/dmrgpy/pyitensor/dmrg.py:336-352). */
