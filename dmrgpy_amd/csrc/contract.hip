@@ -16,6 +16,7 @@
 // with W12 = sum_wm W_i W_{i+1} pre-contracted on the host once per bond.
 #include "dmrg_common.h"
 #include "dmrg_internal.h"
+#include "small_op.h"
 
 #include <algorithm>
 #include <cmath>
@@ -220,34 +221,13 @@ SmallOp upload_smallop(Ctx* c, Slot slot, int prec, const std::vector<double>& w
   s.wcplx = wcplx;
   const size_t es = prec_size(prec);
   auto up = [](size_t b) { return ((b + 255) / 256) * 256; };
-  // compressed rows (real W only)
+  // compressed rows (real W only) and the four waves' row ranges
   std::vector<int> rowptr(Kout + 1, 0), col;
   std::vector<double> val;
-  if (!wcplx) {
-    for (int o = 0; o < Kout; ++o) {
-      for (int i = 0; i < Kin; ++i) {
-        const double v = wre[(size_t)o * Kin + i];
-        if (v != 0.0) {
-          col.push_back(i);
-          val.push_back(v);
-        }
-      }
-      rowptr[o + 1] = (int)col.size();
-    }
-  }
+  if (!wcplx) compress_rows(wre, Kout, Kin, rowptr, col, val);
   s.nnz = (int)col.size();
-  // row ranges of the four waves: contiguous, about nnz / 4 (plus one per
-  // row for its store) each
-  int grp[5] = {0, Kout, Kout, Kout, Kout};
-  {
-    const double total = (double)s.nnz + Kout;
-    int g = 1;
-    double run = 0;
-    for (int o = 0; o < Kout && g < 4; ++o) {
-      run += (rowptr[o + 1] - rowptr[o]) + 1;
-      if (run >= total * g / 4.0) grp[g++] = o + 1;
-    }
-  }
+  int grp[5];
+  wave_groups(rowptr, Kout, grp);
   const size_t nw = (size_t)s.KoutPad * Kin * (wcplx ? 2 : 1);
   const size_t bytes_w = up(nw * es);
   const size_t bytes_o = up(sizeof(int) * (Kin + Kout + Kout + 1 + 5 + col.size()));
@@ -589,10 +569,6 @@ GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int ta
 // ---------------------------------------------------------------------------
 // H_eff
 // ---------------------------------------------------------------------------
-static double wget(const double* W, int i0, int i1, int i2, int i3, int n1, int n2, int n3) {
-  return W[(((size_t)i0 * n1 + i1) * n2 + i2) * n3 + i3];
-}
-
 // Row pitches that keep every H_eff GEMM operand row 16-byte aligned: the
 // T2 intermediate [(a' p')][(wr b)] and, when P*Dr is not a multiple of 4,
 // the contraction-precision copy of theta.
@@ -657,27 +633,8 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   if (d.nsite == 2) {
     const int d1 = d.d1, d2 = d.d2, wm = d.wm, P = d1 * d2;
     const int Kin = wl * P, Kout = P * wr;
-    std::vector<double> wre((size_t)Kout * Kin, 0.0), wim(wc ? (size_t)Kout * Kin : 0, 0.0);
-    const size_t n1 = (size_t)wl * wm * d1 * d1, n2 = (size_t)wm * wr * d2 * d2;
-    for (int a = 0; a < wl; ++a)
-      for (int b = 0; b < wr; ++b)
-        for (int s1o = 0; s1o < d1; ++s1o)
-          for (int s2o = 0; s2o < d2; ++s2o)
-            for (int s1 = 0; s1 < d1; ++s1)
-              for (int s2 = 0; s2 < d2; ++s2) {
-                double re = 0, im = 0;
-                for (int m = 0; m < wm; ++m) {
-                  double ar = wget(d.W1, a, m, s1o, s1, wm, d1, d1);
-                  double br = wget(d.W2, m, b, s2o, s2, wr, d2, d2);
-                  double ai = wc ? wget(d.W1 + n1, a, m, s1o, s1, wm, d1, d1) : 0;
-                  double bi = wc ? wget(d.W2 + n2, m, b, s2o, s2, wr, d2, d2) : 0;
-                  re += ar * br - ai * bi;
-                  im += ar * bi + ai * br;
-                }
-                int o = (s1o * d2 + s2o) * wr + b, i = a * P + s1 * d2 + s2;
-                wre[(size_t)o * Kin + i] = re;
-                if (wc) wim[(size_t)o * Kin + i] = im;
-              }
+    std::vector<double> wre, wim;
+    fuse_two_site(d.W1, d.W2, wc, wl, wm, wr, d1, d2, wre, wim);
     std::vector<int> ioff(Kin), ooff(Kout);
     for (int a = 0; a < wl; ++a)
       for (int p = 0; p < P; ++p) ioff[a * P + p] = a * Dl * P * Dr + p * Dr;
@@ -688,16 +645,8 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   } else {
     const int dd = d.d1, P = dd;
     const int Kin = wl * P, Kout = P * wr;
-    std::vector<double> wre((size_t)Kout * Kin, 0.0), wim(wc ? (size_t)Kout * Kin : 0, 0.0);
-    const size_t n1 = (size_t)wl * wr * dd * dd;
-    for (int a = 0; a < wl; ++a)
-      for (int b = 0; b < wr; ++b)
-        for (int so = 0; so < dd; ++so)
-          for (int s = 0; s < dd; ++s) {
-            int o = so * wr + b, i = a * P + s;
-            wre[(size_t)o * Kin + i] = wget(d.W1, a, b, so, s, wr, dd, dd);
-            if (wc) wim[(size_t)o * Kin + i] = wget(d.W1 + n1, a, b, so, s, wr, dd, dd);
-          }
+    std::vector<double> wre, wim;
+    one_site_op(d.W1, wc, wl, wr, dd, wre, wim);
     std::vector<int> ioff(Kin), ooff(Kout);
     for (int a = 0; a < wl; ++a)
       for (int p = 0; p < P; ++p) ioff[a * P + p] = a * Dl * P * Dr + p * Dr;
