@@ -693,19 +693,7 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   return h;
 }
 
-double heff_flops(const dmrg_heff_desc& d) {
-  const double Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr;
-  double mac;
-  if (d.nsite == 2) {
-    const double d1 = d.d1, d2 = d.d2, wm = d.wm;
-    mac = Dl * Dl * wl * d1 * d2 * Dr + Dl * Dr * d2 * wl * wm * d1 * d1 + Dl * Dr * d1 * wm * wr * d2 * d2 +
-          Dl * d1 * d2 * wr * Dr * Dr;
-  } else {
-    const double dd = d.d1;
-    mac = Dl * Dl * wl * dd * Dr + Dl * Dr * wl * wr * dd * dd + Dl * dd * wr * Dr * Dr;
-  }
-  return (d.field == CPLX ? 8.0 : 2.0) * mac;
-}
+// heff_flops: heff_work.h.
 
 // y += w sum_k p_k <p_k|x> for the penalty projectors of the operator (FP64;
 // the coefficients stay on the device)

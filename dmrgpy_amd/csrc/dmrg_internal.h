@@ -5,6 +5,7 @@
 
 #include "dmrg_common.h"
 #include "../../include/dmrg_hip.h"
+#include "heff_work.h"
 
 namespace dmrg {
 
@@ -84,7 +85,6 @@ void ormtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double*
 void unmtr_lower_rows(Ctx* c, const double* A, int n, const double* tau, double* Et, int keep, int nrefl = -1);
 GemmDesc gdesc(int M, int N, int K, int lda, int ldb, int ldc, int field, int tag);
 
-double heff_flops(const dmrg_heff_desc& d);
 // Upper bound of the workspace bytes heff_prepare + heff_apply request for
 // the descriptor (each slot rounded up to 256 bytes), for the allocation-free
 // entry point's size check
