@@ -660,19 +660,7 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   DMRG_REQUIRE((long long)wl * Dl * h.P * Dr < (1LL << 31) && (long long)Dl * h.P * t2_pitch(wr, Dr) < (1LL << 31),
                "heff: tensor too large for 32-bit offsets");
   h.flops_per_apply = heff_flops(d);
-  {
-    const double np_ = d.field == CPLX ? 2.0 : 1.0, es = (double)prec_size(d.prec);
-    const double n = (double)h.n, t1 = (double)wl * Dl * h.P * Dr, t2 = (double)Dl * h.P * wr * Dr;
-    // FP32 real: the R-GEMM writes y in FP64 itself (no conversion pass)
-    const bool direct64 = d.prec == P32 && d.field == REAL;
-    double b = np_ * es * ((double)wl * Dl * Dl + n + t1     // L-GEMM: L, x in, T1 out
-                           + t1 + t2                          // small operator: T1 in, T2 out
-                           + t2 + (double)wr * Dr * Dr);      // R-GEMM: T2, R in
-    b += np_ * n * (direct64 ? 8.0 : es);                     // R-GEMM: y out
-    if (d.prec != P64) b += np_ * n * (8.0 + es);             // x to the contraction precision
-    if (d.prec != P64 && !direct64) b += np_ * n * (es + 8.0);  // y back to FP64
-    h.bytes_per_apply = b;
-  }
+  h.bytes_per_apply = heff_bytes(d);
   if (d.prec == PB3 && d.field == REAL && smallop_sparse_ok(h.sop)) {
     // environments split once per operator (they are fixed for the whole
     // Krylov solve): L rows [(wl a')][a], R transposed to rows [b'][(wr b)]

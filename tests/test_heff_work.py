@@ -1,10 +1,17 @@
-"""The H_eff FLOP count behind the roofline figures
+"""The H_eff FLOP count and algorithmic bytes behind the roofline figures
 (dmrgpy_amd/csrc/heff_work.h), compiled for the host with g++ and checked
-against a count derived from the contraction chain itself (CPU): each
-pairwise contraction of the chain costs the product of the extents of every
-index its two operands carry (the kept ones and the summed ones), and a MAC
-is 2 FLOPs real, 8 complex.  A dropped step, a wrong extent or a swapped
-site dimension changes the count at one of the shapes below.
+against counts derived from the contraction chain itself (CPU):
+
+* FLOPs: each pairwise contraction of the chain costs the product of the
+  extents of every index its two operands carry (the kept ones and the
+  summed ones); a MAC is 2 FLOPs real, 8 complex;
+* bytes: every operand of every step read once and every result written
+  once -- sizes from the same index sets -- in the contraction precision,
+  plus the FP64 <-> contraction-precision conversions of x and y below FP64
+  (none for y when the FP32 real R-GEMM writes FP64 itself).
+
+A dropped step, a wrong extent or a swapped site dimension changes a count
+at one of the shapes below.
 """
 
 import ctypes
@@ -25,6 +32,12 @@ extern "C" double t_flops(int nsite, int field, int Dl, int Dr, int d1, int d2, 
   d.wl = wl; d.wm = wm; d.wr = wr;
   return dmrg::heff_flops(d);
 }
+extern "C" double t_bytes(int nsite, int field, int prec, int Dl, int Dr, int d1, int d2, int wl, int wm, int wr) {
+  dmrg_heff_desc d = {};
+  d.nsite = nsite; d.field = field; d.prec = prec; d.Dl = Dl; d.Dr = Dr; d.d1 = d1; d.d2 = d2;
+  d.wl = wl; d.wm = wm; d.wr = wr;
+  return dmrg::heff_bytes(d);
+}
 """
 
 # The chains as einsum-style steps (operand index letters; the extents come
@@ -32,6 +45,14 @@ extern "C" double t_flops(int nsite, int field, int Dl, int Dr, int d1, int d2, 
 # a = a (ket), s / t = s1 / s2, u / v = s1' / s2', b = b, q = b'.
 TWO_SITE = [("xpa", "astb"), ("xpstb", "xmus"), ("mputb", "mrvt"), ("rpuvb", "rqb")]
 ONE_SITE = [("xpa", "asb"), ("xpsb", "xrus"), ("rpub", "rqb")]
+
+# The kernels the library runs for the same chains: (inputs, output) read /
+# written through HBM.  The two MPO steps are one pass over T1; the MPO
+# tensors themselves (a few KB) are not counted.
+TWO_SITE_PASSES = [(["xpa", "astb"], "xpstb"),        # L-GEMM: L, x -> T1
+                   (["xpstb"], "rpuvb"),              # MPO pass: T1 -> T2
+                   (["rpuvb", "rqb"], "puvq")]        # R-GEMM: T2, R -> y
+ONE_SITE_PASSES = [(["xpa", "asb"], "xpsb"), (["xpsb"], "rpub"), (["rpub", "rqb"], "puq")]
 
 
 def chain_macs(steps, dims):
@@ -44,8 +65,33 @@ def chain_macs(steps, dims):
     return total
 
 
+def size(idx, dims):
+    n = 1
+    for ch in idx:
+        n *= dims[ch]
+    return n
+
+
+def chain_bytes(passes, dims, field, prec):
+    npl, es = (2 if field else 1), (8 if prec == 0 else 4)
+    x, y = passes[0][0][1], passes[-1][1]
+    total = 0
+    for ins, out in passes:
+        for t in ins:
+            total += npl * es * size(t, dims)
+        if out != y:
+            total += npl * es * size(out, dims)
+    direct64 = prec == 1 and field == 0
+    total += npl * size(y, dims) * (8 if direct64 else es)
+    if prec != 0:
+        total += npl * size(x, dims) * (8 + es)
+        if not direct64:
+            total += npl * size(y, dims) * (es + 8)
+    return total
+
+
 @pytest.fixture(scope="module")
-def flops(tmp_path_factory):
+def lib(tmp_path_factory):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
@@ -57,6 +103,13 @@ def flops(tmp_path_factory):
     lib = ctypes.CDLL(str(so))
     lib.t_flops.argtypes = [ctypes.c_int] * 9
     lib.t_flops.restype = ctypes.c_double
+    lib.t_bytes.argtypes = [ctypes.c_int] * 10
+    lib.t_bytes.restype = ctypes.c_double
+    return lib
+
+
+@pytest.fixture(scope="module")
+def flops(lib):
     return lib.t_flops
 
 
@@ -96,3 +149,15 @@ def test_bulk_hubbard_figure(flops):
     D, d, w = 1024.0, 4.0, 6.0
     assert f == 2 * (2 * w * d ** 2 * D ** 3 + 2 * w ** 2 * d ** 3 * D ** 2)
     assert 4.21e11 < f < 4.23e11
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=["x".join(map(str, s)) for s in SHAPES])
+@pytest.mark.parametrize("field", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_bytes(lib, shape, field, prec):
+    Dl, Dr, d1, d2, wl, wm, wr = shape
+    dims = dict(p=Dl, a=Dl, b=Dr, q=Dr, s=d1, u=d1, t=d2, v=d2, x=wl, m=wm, r=wr)
+    assert lib.t_bytes(2, field, prec, Dl, Dr, d1, d2, wl, wm, wr) == pytest.approx(
+        chain_bytes(TWO_SITE_PASSES, dims, field, prec), rel=1e-15)
+    assert lib.t_bytes(1, field, prec, Dl, Dr, d1, 0, wl, 0, wr) == pytest.approx(
+        chain_bytes(ONE_SITE_PASSES, dims, field, prec), rel=1e-15)
