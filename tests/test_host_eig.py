@@ -180,7 +180,7 @@ def exact_decomposition(k):
     """The k x k test tridiagonal and its eigen-decomposition at 32 digits
     (mpmath), cached per k."""
     if k not in _EXACT:
-        import mpmath
+        mpmath = pytest.importorskip("mpmath")
         rng = np.random.default_rng(1000 + k)
         al = rng.uniform(-4.0, 4.0, k)
         be = rng.uniform(0.05, 2.0, k - 1)
@@ -193,7 +193,7 @@ def exact_decomposition(k):
 @pytest.mark.parametrize("k", [1, 2, 3, 8, 20, 50])
 @pytest.mark.parametrize("coeff", [-0.05, -1.0j * 0.05, -0.025 - 0.5j, -3.0])
 def test_tridiag_expm_col0(lib, k, coeff):
-    import mpmath
+    mpmath = pytest.importorskip("mpmath")
     al, be, lam, Q = exact_decomposition(k)
     c = complex(coeff)
     out = np.empty(2 * k)
