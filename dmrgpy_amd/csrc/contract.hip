@@ -630,6 +630,13 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
   heff_check(d);
   const int Dl = d.Dl, Dr = d.Dr, wl = d.wl, wr = d.wr;
   const bool wc = d.w_complex != 0;
+  {
+    // offsets in the small-op tables are 32-bit: check before building them
+    const long long Pll = d.nsite == 2 ? (long long)d.d1 * d.d2 : (long long)d.d1;
+    const long long t2p = ((long long)wr * Dr + 7) / 8 * 8;   // t2_pitch without int overflow
+    DMRG_REQUIRE((long long)wl * Dl * Pll * Dr < (1LL << 31) && (long long)Dl * Pll * t2p < (1LL << 31),
+                 "heff: tensor too large for 32-bit offsets");
+  }
   if (d.nsite == 2) {
     const int d1 = d.d1, d2 = d.d2, wm = d.wm, P = d1 * d2;
     const int Kin = wl * P, Kout = P * wr;
@@ -656,9 +663,6 @@ HeffOp heff_prepare(Ctx* c, const dmrg_heff_desc& d) {
     h.P = P;
   }
   h.n = (long long)Dl * h.P * Dr;
-  // offsets in the small-op tables are 32-bit
-  DMRG_REQUIRE((long long)wl * Dl * h.P * Dr < (1LL << 31) && (long long)Dl * h.P * t2_pitch(wr, Dr) < (1LL << 31),
-               "heff: tensor too large for 32-bit offsets");
   h.flops_per_apply = heff_flops(d);
   h.bytes_per_apply = heff_bytes(d);
   if (d.prec == PB3 && d.field == REAL && smallop_sparse_ok(h.sop)) {
