@@ -822,6 +822,8 @@ void env_update(Ctx* c, const dmrg_env_desc& e) {
   if (e.dir == 0) {
     // E1[(wl a')][(s b)] = L[(wl a')][a] * ket[a][(s b)]
     const long long n1 = (long long)wl * Dlb * dd * Dr, n2 = (long long)wr * Dlb * dd * Dr;
+    // the MPO pass's offset tables (channel strides, below n1 / n2) are 32-bit
+    DMRG_REQUIRE(n1 < (1LL << 31) && n2 < (1LL << 31), "env: tensor too large for 32-bit offsets");
     void* E1 = ws_get(c, WS_ENV1, n1 * np * es);
     void* E2 = ws_get(c, WS_ENV2, n2 * np * es);
     GemmDesc g = gdesc(wl * Dlb, dd * Dr, Dl, ldIn, dd * Dr, dd * Dr, e.field, 3);
@@ -844,6 +846,7 @@ void env_update(Ctx* c, const dmrg_env_desc& e) {
   } else {
     // F1[wr][(a s)][b'] = ket[(a s)][b] * R[wr][b][b']
     const long long n1 = (long long)wr * Dl * dd * Drb, n2 = (long long)wl * Dl * dd * Drb;
+    DMRG_REQUIRE(n1 < (1LL << 31) && n2 < (1LL << 31), "env: tensor too large for 32-bit offsets");
     void* F1 = ws_get(c, WS_ENV1, n1 * np * es);
     void* F2 = ws_get(c, WS_ENV2, n2 * np * es);
     GemmDesc g = gdesc(Dl * dd, Drb, Dr, Dr, ldIn, Drb, e.field, 3);
