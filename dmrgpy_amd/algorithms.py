@@ -353,11 +353,11 @@ def random_bond_dims(dims, m):
     return bonds
 
 
-def random_mps(eng, dims, m, field=1, rng=None):
-    """randomMPS (mpsalgebra.py:315-353): per site ``randn(*shape) +
-    1j*randn(*shape)`` from the legacy global numpy generator (or ``rng``),
-    reference boundary shapes; field 0 keeps the real part of the same
-    draws.  Canonicalised to centre N on the device and normalised."""
+def random_mps_arrays(dims, m, field=1, rng=None):
+    """The host site arrays of randomMPS (mpsalgebra.py:315-353), each
+    [Dl][d][Dr]: per site ``randn(*shape) + 1j*randn(*shape)`` from the
+    legacy global numpy generator (or ``rng``) with the reference's boundary
+    shapes; field 0 keeps the real part of the same draws."""
     rs = np.random if rng is None else rng
     n = len(dims)
     bonds = random_bond_dims(dims, m)
@@ -369,6 +369,14 @@ def random_mps(eng, dims, m, field=1, rng=None):
         Dr = bonds[i] if i < n - 1 else 1
         a = a.reshape(Dl, dims[i], Dr)
         arrays.append(a if field else a.real)
+    return arrays
+
+
+def random_mps(eng, dims, m, field=1, rng=None):
+    """randomMPS (mpsalgebra.py:315-353): the draws of random_mps_arrays,
+    canonicalised to centre N on the device and normalised."""
+    arrays = random_mps_arrays(dims, m, field, rng)
+    n = len(dims)
     psi = DeviceMPS.from_numpy(arrays, eng.device, field=field, center=1)
     position(eng, psi, n)
     normalize(eng, psi)
