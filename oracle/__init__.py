@@ -27,8 +27,10 @@ follows):
 * ``mps``        random MPS, canonical moves, inner -- pinned by dense vectors
 * ``dmrg``       sweeps                 -- pinned by ED, closed forms and the
   golden energies written from the reference (tests/golden/)
-* ``tdvp``       TDVP step              -- pinned by dense expm at N<=8 and the
-  golden correlators from the reference
+* ``tdvp``       TDVP step, quench, evolve-and-measure -- pinned by dense
+  expm at N<=8 and the golden correlators from the reference
+  (tests/test_oracle.py::test_tdvp_golden_quench,
+  test_tdvp_golden_evolve_and_measure)
 * ``ed``         exact diagonalisation and the XX-chain free-fermion closed
   forms (``free_fermion_xx``, ``free_fermion_xx_szsz``) -- pinned by the
   reference's ED values (dimer, N=3, N=10, TFIM N=4) and by each other at

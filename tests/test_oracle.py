@@ -192,6 +192,26 @@ def test_tdvp_matches_dense_expm():
     assert abs(abs(np.vdot(psi.to_dense(), exact)) - 1) < 1e-10
 
 
+def test_tdvp_golden_evolve_and_measure():
+    """The oracle's evolve_and_measure (chain.py:544-564) against the
+    reference's correlators: from the seeded ground state, S^x_2 applied by
+    zip-up at the session's cutoff / maxm, then <S^z_4> before each step."""
+    t = TGOLD[0]
+    codes = t["codes"]
+    terms = [(complex(*x[0]), [tuple(f) for f in x[1]]) for x in t["terms"]]
+    Ws = mpo.to_mpo(codes, terms)
+    np.random.seed(t["seed"])
+    psi = mps.random_mps([2] * len(codes), t["maxm"])
+    for _ in range(t["nsweeps"]):
+        dmrg.sweep(psi, Ws, t["maxm"], t["cutoff"], 2)
+    assert t["evolve"]["apply"] == "Sx_2" and t["evolve"]["measure"] == "Sz_4"
+    phi = mps.apply_mpo(mpo.to_mpo(codes, [(1.0, [("Sx", 2)])]), psi, t["cutoff"], t["maxm"])
+    A = mpo.to_mpo(codes, [(1.0, [("Sz", 4)])])
+    corr, _ = tdvp.evolve_and_measure(phi, Ws, A, t["evolve"]["nt"], t["evolve"]["dt"], t["cutoff"], t["maxm"])
+    ref = np.array([complex(*z) for z in t["evolve"]["corr"]])
+    assert np.abs(np.array(corr) - ref).max() < 1e-12
+
+
 def test_tdvp_golden_quench():
     t = TGOLD[0]
     codes = t["codes"]
